@@ -1,0 +1,17 @@
+"""MI355X (gfx950) rebuild of Ocean's estimation-based SpGEMM.
+
+The product is ``libocean.so`` (hand-written HIP kernels and the C++
+orchestrator behind the C ABI of ``include/ocean_capi.h``) and the
+``spgemm`` CLI.  ``research_amd.ocean`` is a ctypes mirror of that ABI for
+Python callers and tests.
+"""
+from .ocean import (  # noqa: F401
+    CSR,
+    Config,
+    DeviceCSR,
+    DeviceBuffer,
+    OceanError,
+    SpGEMM,
+    lib,
+    library_path,
+)

@@ -1,0 +1,1013 @@
+// gfx950 accumulator kernels of the SpGEMM hot path:
+//   ESC            rows with <= 16/32/64 products, one product per lane
+//   LDS hash       symbolic (set) and numeric (col -> FP64 sum) tables with
+//                  linear probing on LDS compare-and-swap, numeric rows are
+//                  sorted in LDS (radix) and streamed to their final place
+//   LDS dense      bitmap (symbolic) / FP64 array + bitmap (numeric) over
+//                  the row's column span; output is produced in order
+//   windowed       persistent kernels for rows that fit neither: the column
+//                  span is swept in LDS-sized windows with per-A-entry
+//                  cursors into the (sorted) B rows
+// All accumulation is in FP64; FP32 inputs/outputs are converted at the
+// load/store.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dev_common.hpp"
+#include "launch.hpp"
+
+namespace oc {
+
+#define HIP_TRY(call)                                                         \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                    \
+                hipGetErrorString(e_));                                       \
+      return OC_ERR_HIP;                                                      \
+    }                                                                         \
+  } while (0)
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) {
+  return (uint32_t)((a + b - 1) / b);
+}
+
+template <typename T>
+struct KArgs {
+  const uint32_t *__restrict__ a_rp;
+  const uint32_t *__restrict__ a_ci;
+  const T *__restrict__ a_v;
+  const uint32_t *__restrict__ b_rp;
+  const uint32_t *__restrict__ b_ci;
+  const T *__restrict__ b_v;
+  uint32_t b_cols;
+  const uint32_t *__restrict__ products;
+  const uint32_t *__restrict__ max_b;
+  const uint32_t *__restrict__ left;
+  const uint32_t *__restrict__ right;
+};
+
+template <typename T>
+struct KOut {
+  const uint64_t *off;
+  uint32_t *cols;
+  T *vals;
+  uint32_t *nnz;
+  const uint32_t *cap;
+  uint32_t *ovf_rows;
+  uint32_t *ovf_count;
+};
+
+template <typename T>
+static KArgs<T> kargs(const AccArgs &a) {
+  KArgs<T> k;
+  k.a_rp = a.A.rp;
+  k.a_ci = a.A.ci;
+  k.a_v = (const T *)a.A.v;
+  k.b_rp = a.B.rp;
+  k.b_ci = a.B.ci;
+  k.b_v = (const T *)a.B.v;
+  k.b_cols = (uint32_t)a.B.cols;
+  k.products = a.products;
+  k.max_b = a.max_b;
+  k.left = a.left;
+  k.right = a.right;
+  return k;
+}
+
+template <typename T>
+static KOut<T> kout(const NumOut &o) {
+  KOut<T> k;
+  k.off = o.off;
+  k.cols = o.cols;
+  k.vals = (T *)o.vals;
+  k.nnz = o.nnz;
+  k.cap = o.cap;
+  k.ovf_rows = o.ovf_rows;
+  k.ovf_count = o.ovf_count;
+  return k;
+}
+
+static int key_bits_for(uint64_t cols) {
+  int b = 0;
+  uint64_t x = cols > 0 ? cols - 1 : 0;
+  while (x) {
+    ++b;
+    x >>= 1;
+  }
+  return b;
+}
+
+// Lanes per A entry for a row (power of two <= 64): about one lane per
+// product of the average referenced B row, widened while the A row leaves
+// whole groups idle.  Same spirit as localLoadBalance
+// (kernels/AccumulatorCommon.cuh:67-110).
+__device__ __forceinline__ int pick_group(uint32_t nA, uint32_t prod,
+                                          uint32_t maxb, int bs) {
+  if (nA == 0) return 1;
+  uint32_t avg = prod / nA;
+  if (avg < 1) avg = 1;
+  int g = 1;
+  while (g < 64 && (uint32_t)g < avg) g <<= 1;
+  while (g < 64 && (uint32_t)(bs / g) > nA && (uint32_t)g < maxb) g <<= 1;
+  return g;
+}
+
+__device__ __forceinline__ uint32_t lds_load(uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void lds_add_f64(double *p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// =================================================================== ESC
+// One product per lane of an N-lane group; the group ranks its N
+// (col, product) pairs by compare-counting and sums equal columns in
+// product order, exactly as countSortCompactFused / ESCKernel
+// (kernels/AccumulatorESC.cuh:12-127).  Owners of products are found by a
+// prefix sum of the B row lengths and a binary search with shuffles.
+template <int N, typename T>
+__global__ __launch_bounds__(256) void k_esc(KArgs<T> a, uint32_t rows,
+                                             KOut<T> o) {
+  __shared__ uint32_t sk[256];
+  __shared__ double sv[256];
+  const int sub = threadIdx.x / N, lig = threadIdx.x % N;
+  uint32_t *mk = sk + sub * N;
+  double *mv = sv + sub * N;
+  const uint32_t per_block = 256 / N;
+  for (uint32_t r = blockIdx.x * per_block + sub; r < rows;
+       r += gridDim.x * per_block) {
+    const uint32_t p = a.products[r];
+    if (p > (uint32_t)N) continue;
+    const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
+    uint32_t col = kEmpty;
+    double val = 0.0;
+    uint32_t base = 0;
+    for (uint32_t c0 = a0; c0 < a1 && base < p; c0 += N) {
+      const uint32_t j = c0 + lig;
+      uint32_t L = 0, bs = 0;
+      double av = 0.0;
+      if (j < a1) {
+        const uint32_t k = a.a_ci[j];
+        bs = a.b_rp[k];
+        L = a.b_rp[k + 1] - bs;
+        av = (double)a.a_v[j];
+      }
+      const uint32_t P = group_incl_sum(L, N);
+      const uint32_t tot = __shfl(P, N - 1, N);
+      const int qq = lig - (int)base;
+      int lo = 0, hi = N - 1;
+#pragma unroll
+      for (int s = N; s > 1; s >>= 1) {
+        int mid = (lo + hi) >> 1;
+        int pm = (int)__shfl(P, mid, N);
+        if (pm > qq) hi = mid;
+        else lo = mid + 1;
+      }
+      const int owner = lo < N ? lo : N - 1;
+      const uint32_t prev = __shfl(P, owner > 0 ? owner - 1 : 0, N);
+      const uint32_t obs = __shfl(bs, owner, N);
+      const double oav = __shfl(av, owner, N);
+      if (qq >= 0 && (uint32_t)qq < tot) {
+        const uint32_t idx = obs + ((uint32_t)qq - (owner > 0 ? prev : 0u));
+        col = a.b_ci[idx];
+        val = oav * (double)a.b_v[idx];
+      }
+      base += tot;
+    }
+    mk[lig] = col;
+    mv[lig] = val;
+    wave_sync();
+    double sum = 0.0;
+    bool lead = col != kEmpty;
+    uint64_t less = 0;
+#pragma unroll 4
+    for (int k = 0; k < N; ++k) {
+      const uint32_t kk = mk[k];
+      if (kk == col) {
+        sum += mv[k];
+        if (k < lig) lead = false;
+      }
+      if (kk < col) less |= 1ull << k;
+    }
+    const uint64_t lm = group_ballot(lead, N);
+    const uint32_t pos = (uint32_t)__popcll(less & lm);
+    const uint32_t n = (uint32_t)__popcll(lm);
+    const uint64_t off = o.off[r];
+    if (lead) {
+      o.cols[off + pos] = col;
+      o.vals[off + pos] = (T)sum;
+    }
+    if (lig == 0) o.nnz[r] = n;
+    wave_sync();
+  }
+}
+
+int launch_esc(int esc_n, const AccArgs &a, const NumOut &o, hipStream_t s) {
+  const uint32_t rows = (uint32_t)a.A.rows;
+  if (rows == 0) return OC_OK;
+  const uint32_t per_block = 256 / esc_n;
+  dim3 grid(std::min<uint32_t>(cdiv(rows, per_block), 1u << 18));
+#define GO(N, T) \
+  hipLaunchKernelGGL((k_esc<N, T>), grid, dim3(256), 0, s, kargs<T>(a), rows, kout<T>(o))
+  if (a.A.vtype == OC_F64) {
+    if (esc_n == 16) GO(16, double);
+    else if (esc_n == 32) GO(32, double);
+    else if (esc_n == 64) GO(64, double);
+    else goto bad;
+  } else {
+    if (esc_n == 16) GO(16, float);
+    else if (esc_n == 32) GO(32, float);
+    else if (esc_n == 64) GO(64, float);
+    else goto bad;
+  }
+#undef GO
+  LAUNCH_CHECK();
+  return OC_OK;
+bad:
+  set_error("ESC kernel size %d unsupported", esc_n);
+  return OC_ERR_INVALID;
+}
+
+// ========================================================== symbolic hash
+// Exact row nnz from an LDS hash set (kernels/AccumulatorHash.cuh:11-152).
+// With limit < S (estimated-compaction tables) a row whose occupancy
+// reaches the limit, or whose probe sequence wraps, is listed as overflow
+// and recounted by the windowed kernel.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
+                                                 const uint32_t *__restrict__ rows,
+                                                 uint32_t S, uint32_t limit,
+                                                 uint32_t *__restrict__ nnz,
+                                                 uint32_t *ovf_rows,
+                                                 uint32_t *ovf_count) {
+  extern __shared__ uint32_t lds_u32[];
+  uint32_t *hdr = lds_u32;
+  uint32_t *tab = lds_u32 + 64;
+  const uint32_t r = rows[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < S; i += BS) tab[i] = kEmpty;
+  if (threadIdx.x < 4) hdr[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
+  const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
+  const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
+  const bool track = limit < S;
+  uint32_t mine = 0;
+  bool ovf = false;
+  for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
+    if (track && lds_load(&hdr[0]) >= limit) {
+      ovf = true;
+      break;
+    }
+    const uint32_t k = a.a_ci[j];
+    const uint32_t be = a.b_rp[k + 1];
+    for (uint32_t q = a.b_rp[k] + lig; q < be; q += G) {
+      const uint32_t c = a.b_ci[q];
+      uint32_t slot = hash_slot(c, S);
+      uint32_t probes = 0;
+      for (; probes < S; ++probes) {
+        const uint32_t v = tab[slot];
+        if (v == c) break;
+        if (v == kEmpty) {
+          const uint32_t old = atomicCAS(&tab[slot], kEmpty, c);
+          if (old == kEmpty) {
+            ++mine;
+            if (track) atomicAdd(&hdr[0], 1u);
+            break;
+          }
+          if (old == c) break;
+        }
+        if (++slot == S) slot = 0;
+      }
+      if (probes == S) {
+        ovf = true;
+        break;
+      }
+    }
+  }
+  if (ovf) hdr[1] = 1;
+  atomicAdd(&hdr[2], mine);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (hdr[1]) ovf_rows[atomicAdd(ovf_count, 1u)] = r;
+    else nnz[r] = hdr[2];
+  }
+}
+
+// ========================================================= symbolic dense
+// Exact row nnz from an LDS bitmap over [left, right]
+// (kernels/AccumulatorDense.cuh:43-126).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
+                                                  const uint32_t *__restrict__ rows,
+                                                  uint32_t *__restrict__ nnz) {
+  extern __shared__ uint32_t lds_u32[];
+  uint32_t *hdr = lds_u32;
+  uint32_t *bits = lds_u32 + 64;
+  const uint32_t r = rows[blockIdx.x];
+  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
+  const uint32_t lo = a.left[r] & ~31u, hi = a.right[r];
+  if (a1 == a0 || a.products[r] == 0 || hi < a.left[r]) {
+    if (threadIdx.x == 0) nnz[r] = 0;
+    return;
+  }
+  const uint32_t nwords = ((hi - lo) >> 5) + 1;
+  for (uint32_t i = threadIdx.x; i < nwords; i += BS) bits[i] = 0;
+  if (threadIdx.x == 0) hdr[0] = 0;
+  __syncthreads();
+  const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
+  const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
+  for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
+    const uint32_t k = a.a_ci[j];
+    const uint32_t be = a.b_rp[k + 1];
+    for (uint32_t q = a.b_rp[k] + lig; q < be; q += G) {
+      const uint32_t c = a.b_ci[q] - lo;
+      atomicOr(&bits[c >> 5], 1u << (c & 31));
+    }
+  }
+  __syncthreads();
+  uint32_t cnt = 0;
+  for (uint32_t i = threadIdx.x; i < nwords; i += BS) cnt += __popc(bits[i]);
+  cnt = group_sum(cnt, kWave);
+  if (lane_id() == 0) atomicAdd(&hdr[0], cnt);
+  __syncthreads();
+  if (threadIdx.x == 0) nnz[r] = hdr[0];
+}
+
+// ========================================================== numeric hash
+// C row values in an LDS hash table, FP64 sums via LDS atomics
+// (kernels/AccumulatorHash.cuh:509-651), then compaction, LDS radix sort
+// of (col, slot) and a streamed, ordered write of the row.  EST: the row's
+// output capacity is cap[r]; rows whose table fills past 2/3 or beyond
+// their capacity are listed as overflow (kernels/AccumulatorHash.cuh:
+// 605-623) and recomputed by the windowed kernel.
+struct HashLayout {
+  uint32_t S, nmax;
+  uint32_t off_keys, off_vals, off_ck, off_ci, off_cnt, bytes;
+};
+
+__host__ __device__ inline HashLayout hash_layout(uint32_t S, uint32_t nwaves) {
+  HashLayout L;
+  L.S = S;
+  L.nmax = 2 * S / 3 + 1;
+  L.off_keys = 1280;
+  uint32_t p = L.off_keys + 4 * S + 8;
+  p = (p + 7) & ~7u;
+  L.off_vals = p;
+  p += 8 * S;
+  L.off_ck = p;
+  p += 4 * L.nmax;
+  L.off_ci = p;
+  p += 2 * L.nmax;
+  p = (p + 3) & ~3u;
+  L.off_cnt = p;
+  p += nwaves * 512;
+  L.bytes = p;
+  return L;
+}
+
+template <int BS, typename T, bool EST>
+__global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
+                                                 const uint32_t *__restrict__ rows,
+                                                 HashLayout L, KOut<T> o,
+                                                 int key_bits) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
+  uint32_t *dsum = reinterpret_cast<uint32_t *>(smem + 256);
+  uint32_t *keys = reinterpret_cast<uint32_t *>(smem + L.off_keys);
+  double *vals = reinterpret_cast<double *>(smem + L.off_vals);
+  uint32_t *ck = reinterpret_cast<uint32_t *>(smem + L.off_ck);
+  uint16_t *ci = reinterpret_cast<uint16_t *>(smem + L.off_ci);
+  uint16_t *cnt = reinterpret_cast<uint16_t *>(smem + L.off_cnt);
+  const uint32_t S = L.S;
+  const uint32_t r = rows[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < S; i += BS) {
+    keys[i] = kEmpty;
+    vals[i] = 0.0;
+  }
+  if (threadIdx.x < 4) hdr[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
+  const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
+  const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
+  uint32_t limit = S;
+  if (EST) limit = min(L.nmax - 1, o.cap[r]);
+  bool ovf = false;
+  for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
+    if (EST && lds_load(&hdr[0]) >= limit) {
+      ovf = true;
+      break;
+    }
+    const uint32_t k = a.a_ci[j];
+    const double av = (double)a.a_v[j];
+    const uint32_t be = a.b_rp[k + 1];
+    for (uint32_t q = a.b_rp[k] + lig; q < be; q += G) {
+      const uint32_t c = a.b_ci[q];
+      const double pv = av * (double)a.b_v[q];
+      uint32_t slot = hash_slot(c, S);
+      uint32_t probes = 0;
+      for (; probes < S; ++probes) {
+        const uint32_t v = keys[slot];
+        if (v == c) break;
+        if (v == kEmpty) {
+          const uint32_t old = atomicCAS(&keys[slot], kEmpty, c);
+          if (old == kEmpty) {
+            if (EST) atomicAdd(&hdr[0], 1u);
+            break;
+          }
+          if (old == c) break;
+        }
+        if (++slot == S) slot = 0;
+      }
+      if (probes == S) {
+        ovf = true;
+        break;
+      }
+      lds_add_f64(&vals[slot], pv);
+    }
+  }
+  if (ovf) hdr[1] = 1;
+  __syncthreads();
+  // compaction of occupied slots
+  if (!hdr[1]) {
+    for (uint32_t i = threadIdx.x; i < S; i += BS) {
+      const uint32_t kk = keys[i];
+      if (kk != kEmpty) {
+        const uint32_t pos = atomicAdd(&hdr[2], 1u);
+        if (pos < L.nmax) {
+          ck[pos] = kk;
+          ci[pos] = (uint16_t)i;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t n = hdr[2];
+  const bool overflow = hdr[1] || n > L.nmax || (EST && n > o.cap[r]);
+  if (overflow) {
+    if (threadIdx.x == 0) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
+    return;
+  }
+  uint32_t *k1 = keys;
+  uint16_t *i1 = reinterpret_cast<uint16_t *>(keys + L.nmax);
+  const int parity = block_radix_sort(ck, ci, k1, i1, cnt, dsum, (int)n, key_bits);
+  const uint32_t *sk = parity ? k1 : ck;
+  const uint16_t *si = parity ? i1 : ci;
+  const uint64_t off = o.off[r];
+  for (uint32_t i = threadIdx.x; i < n; i += BS) {
+    o.cols[off + i] = sk[i];
+    o.vals[off + i] = (T)vals[si[i]];
+  }
+  if (o.nnz && threadIdx.x == 0) o.nnz[r] = n;
+}
+
+// ========================================================= numeric dense
+// C row values over the LDS span [left, right] (FP64 sums + bitmap), then
+// an ordered compaction (kernels/AccumulatorDense.cuh:132-346).  EST: rows
+// whose count exceeds cap[r] are listed as overflow
+// (kernels/AccumulatorDense.cuh:327-339).
+template <int BS, typename T, bool EST>
+__global__ __launch_bounds__(BS) void k_num_dense(KArgs<T> a,
+                                                  const uint32_t *__restrict__ rows,
+                                                  KOut<T> o) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
+  const uint32_t r = rows[blockIdx.x];
+  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
+  const uint32_t lo = a.left[r] & ~31u, hi = a.right[r];
+  if (a1 == a0 || a.products[r] == 0 || hi < a.left[r]) {
+    if (threadIdx.x == 0 && o.nnz) o.nnz[r] = 0;
+    return;
+  }
+  const uint32_t nwords = ((hi - lo) >> 5) + 1;
+  double *vals = reinterpret_cast<double *>(smem + 256);
+  uint32_t *bits = reinterpret_cast<uint32_t *>(smem + 256 + (size_t)nwords * 256);
+  for (uint32_t i = threadIdx.x; i < nwords * 32; i += BS) vals[i] = 0.0;
+  for (uint32_t i = threadIdx.x; i < nwords; i += BS) bits[i] = 0;
+  __syncthreads();
+  const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
+  const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
+  for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
+    const uint32_t k = a.a_ci[j];
+    const double av = (double)a.a_v[j];
+    const uint32_t be = a.b_rp[k + 1];
+    for (uint32_t q = a.b_rp[k] + lig; q < be; q += G) {
+      const uint32_t c = a.b_ci[q] - lo;
+      lds_add_f64(&vals[c], av * (double)a.b_v[q]);
+      atomicOr(&bits[c >> 5], 1u << (c & 31));
+    }
+  }
+  __syncthreads();
+  const uint32_t wpt = (nwords + BS - 1) / BS;
+  const uint32_t w0 = min(nwords, threadIdx.x * wpt), w1 = min(nwords, w0 + wpt);
+  uint32_t mycnt = 0;
+  for (uint32_t w = w0; w < w1; ++w) mycnt += __popc(bits[w]);
+  uint32_t total;
+  const uint32_t base = block_excl_sum(mycnt, hdr + 8, &total);
+  if (EST && total > o.cap[r]) {
+    if (threadIdx.x == 0) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
+    return;
+  }
+  uint64_t pos = o.off[r] + base;
+  for (uint32_t w = w0; w < w1; ++w) {
+    uint32_t m = bits[w];
+    while (m) {
+      const int b = __ffs(m) - 1;
+      m &= m - 1;
+      const uint32_t c = w * 32 + b;
+      o.cols[pos] = lo + c;
+      o.vals[pos] = (T)vals[c];
+      ++pos;
+    }
+  }
+  if (o.nnz && threadIdx.x == 0) o.nnz[r] = total;
+}
+
+// ============================================================== windowed
+// Persistent kernels (one 1024-thread workgroup per CU) over the rows of
+// a bin, taken from an atomic counter.  The row's column span is swept in
+// windows of W columns; each A entry keeps a cursor into its B row, whose
+// columns ascend, so a window consumes a prefix of what is left of every B
+// row.  The cursor lives with the lane group that owns the A entry (LDS for
+// rows up to 4096 entries, else a per-workgroup global scratch slot).
+// Numeric windows hold FP64 sums + a bitmap and are written out in order
+// (kernels/AccumulatorDense.cuh:610-806); symbolic windows only count.
+constexpr int kWinBlock = 1024;
+constexpr uint32_t kWinLdsCursors = 4096;
+
+template <typename T, bool SYM>
+__global__ __launch_bounds__(kWinBlock) void k_window(
+    KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows,
+    uint32_t *counter, uint32_t W, KOut<T> o, uint32_t *__restrict__ nnz_sym,
+    uint32_t *__restrict__ scratch, uint32_t max_nA) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
+  uint32_t *lcur = reinterpret_cast<uint32_t *>(smem + 256);
+  uint32_t *bits = reinterpret_cast<uint32_t *>(smem + 256 + kWinLdsCursors * 4);
+  double *vals = reinterpret_cast<double *>(smem + 256 + kWinLdsCursors * 4 + (W / 32) * 4);
+  for (uint32_t i = threadIdx.x; i < W / 32; i += kWinBlock) bits[i] = 0;
+  if (!SYM)
+    for (uint32_t i = threadIdx.x; i < W; i += kWinBlock) vals[i] = 0.0;
+  __syncthreads();
+  for (;;) {
+    if (threadIdx.x == 0) hdr[0] = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t idx = hdr[0];
+    __syncthreads();
+    if (idx >= nrows) break;
+    const uint32_t r = rows[idx];
+    const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1], nA = a1 - a0;
+    const uint32_t left = a.left[r], hi = a.right[r];
+    if (nA == 0 || a.products[r] == 0 || hi < left) {
+      if (threadIdx.x == 0) {
+        if (SYM) nnz_sym[r] = 0;
+        else if (o.nnz) o.nnz[r] = 0;
+      }
+      continue;
+    }
+    const uint32_t lo = left & ~31u;
+    uint32_t *cur = nA <= kWinLdsCursors ? lcur : scratch + (uint64_t)blockIdx.x * max_nA;
+    const uint32_t nwin = (hi - lo) / W + 1;
+    const uint32_t per = (uint32_t)(((uint64_t)a.products[r] + (uint64_t)nA * nwin - 1) /
+                                    ((uint64_t)nA * nwin));
+    int G = 1;
+    while (G < 64 && (uint32_t)G < per) G <<= 1;
+    const int ngroups = kWinBlock / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
+    uint64_t total = 0;
+    const uint64_t off = SYM ? 0 : o.off[r];
+    for (uint32_t w = 0; w < nwin; ++w) {
+      const uint64_t w0 = (uint64_t)lo + (uint64_t)w * W;
+      const uint64_t w1 = min((uint64_t)hi + 1, w0 + W);
+      for (uint32_t jj = gid; jj < nA; jj += ngroups) {
+        const uint32_t k = a.a_ci[a0 + jj];
+        const uint32_t e = a.b_rp[k + 1];
+        uint32_t c = w == 0 ? a.b_rp[k] : cur[jj];
+        double av = 0.0;
+        if (!SYM) av = (double)a.a_v[a0 + jj];
+        for (;;) {
+          const uint32_t q = c + lig;
+          const uint32_t col = q < e ? a.b_ci[q] : kEmpty;
+          const bool valid = q < e && (uint64_t)col < w1;
+          if (valid) {
+            const uint32_t x = (uint32_t)(col - w0);
+            atomicOr(&bits[x >> 5], 1u << (x & 31));
+            if (!SYM) lds_add_f64(&vals[x], av * (double)a.b_v[q]);
+          }
+          const uint32_t nv = (uint32_t)__popcll(group_ballot(valid, G));
+          c += nv;
+          if (nv < (uint32_t)G) break;
+        }
+        if (lig == 0) cur[jj] = c;
+      }
+      __syncthreads();
+      const uint32_t nwords = (uint32_t)((w1 - w0 + 31) / 32);
+      const uint32_t wpt = (nwords + kWinBlock - 1) / kWinBlock;
+      const uint32_t b0 = min(nwords, threadIdx.x * wpt), b1 = min(nwords, b0 + wpt);
+      uint32_t mycnt = 0;
+      for (uint32_t i = b0; i < b1; ++i) mycnt += __popc(bits[i]);
+      uint32_t wtotal;
+      const uint32_t base = block_excl_sum(mycnt, hdr + 8, &wtotal);
+      if (SYM) {
+        for (uint32_t i = b0; i < b1; ++i) bits[i] = 0;
+      } else {
+        uint64_t pos = off + total + base;
+        for (uint32_t i = b0; i < b1; ++i) {
+          uint32_t m = bits[i];
+          bits[i] = 0;
+          while (m) {
+            const int b = __ffs(m) - 1;
+            m &= m - 1;
+            const uint32_t x = i * 32 + b;
+            o.cols[pos] = (uint32_t)(w0 + x);
+            o.vals[pos] = (T)vals[x];
+            vals[x] = 0.0;
+            ++pos;
+          }
+        }
+      }
+      total += wtotal;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (SYM) nnz_sym[r] = (uint32_t)total;
+      else if (o.nnz) o.nnz[r] = (uint32_t)total;
+    }
+  }
+}
+
+size_t window_scratch_bytes(uint32_t max_a_nnz) {
+  const Tables &t = tables();
+  uint64_t grid = std::max<uint32_t>(1, t.num_cu);
+  uint64_t per = max_a_nnz > kWinLdsCursors ? max_a_nnz : 0;
+  return (size_t)(grid * per * 4 + 256);
+}
+
+static size_t window_lds(uint32_t W, bool sym) {
+  return 256 + kWinLdsCursors * 4 + (W / 32) * 4 + (sym ? 0 : (size_t)W * 8);
+}
+
+template <typename K>
+static int set_lds(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    HIP_TRY(hipFuncSetAttribute((const void *)kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+  return OC_OK;
+}
+
+template <typename T, bool SYM>
+static int window_go(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
+                     uint32_t max_a_nnz, const NumOut &o, uint32_t *nnz_sym,
+                     void *scratch, hipStream_t s) {
+  if (nrows == 0) return OC_OK;
+  const Tables &t = tables();
+  const uint32_t W = SYM ? t.win_sym_cols : t.win_num_cols;
+  const size_t lds = window_lds(W, SYM);
+  if (lds > t.lds_max_block) {
+    set_error("window kernel needs %zu B of LDS, device allows %u", lds, t.lds_max_block);
+    return OC_ERR_INVALID;
+  }
+  OC_TRY(set_lds(k_window<T, SYM>, lds));
+  uint32_t *counter = (uint32_t *)scratch;
+  uint32_t *cur = counter + 64;
+  HIP_TRY(hipMemsetAsync(counter, 0, 4, s));
+  const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu));
+  KOut<T> ko = kout<T>(o);
+  hipLaunchKernelGGL((k_window<T, SYM>), dim3(grid), dim3(kWinBlock), lds, s,
+                     kargs<T>(a), rows, nrows, counter, W, ko, nnz_sym, cur,
+                     max_a_nnz);
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
+int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
+                      uint32_t max_a_nnz, uint32_t *nnz, void *scratch,
+                      hipStream_t s) {
+  NumOut o;
+  std::memset(&o, 0, sizeof(o));
+  return window_go<float, true>(a, rows, nrows, max_a_nnz, o, nnz, scratch, s);
+}
+
+int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
+                      uint32_t max_a_nnz, const NumOut &o, void *scratch,
+                      hipStream_t s) {
+  if (a.A.vtype == OC_F64)
+    return window_go<double, false>(a, rows, nrows, max_a_nnz, o, nullptr, scratch, s);
+  return window_go<float, false>(a, rows, nrows, max_a_nnz, o, nullptr, scratch, s);
+}
+
+// ------------------------------------------------------------ launchers
+// Block sizes per bin (gfx950): 64, 64, 128, 256, 512, 512, 1024.
+#define BIN_DISPATCH(bin, MACRO)          \
+  switch (bin) {                          \
+    case 0: MACRO(64); break;             \
+    case 1: MACRO(64); break;             \
+    case 2: MACRO(128); break;            \
+    case 3: MACRO(256); break;            \
+    case 4: MACRO(512); break;            \
+    case 5: MACRO(512); break;            \
+    case 6: MACRO(1024); break;           \
+    default:                              \
+      set_error("bad bin %d", bin);       \
+      return OC_ERR_INVALID;              \
+  }
+
+int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
+                    uint32_t nrows, uint32_t limit, uint32_t *nnz,
+                    uint32_t *ovf_rows, uint32_t *ovf_count, hipStream_t s) {
+  if (nrows == 0) return OC_OK;
+  const Tables &t = tables();
+  const uint32_t S = t.hash_sym[bin];
+  const size_t lds = 256 + (size_t)S * 4;
+  KArgs<float> k = kargs<float>(a);
+#define GO(BS)                                                                \
+  do {                                                                        \
+    OC_TRY(set_lds(k_sym_hash<BS>, lds));                                     \
+    hipLaunchKernelGGL(k_sym_hash<BS>, dim3(nrows), dim3(BS), lds, s, k, rows, \
+                       S, std::min(limit, S), nnz, ovf_rows, ovf_count);      \
+  } while (0)
+  BIN_DISPATCH(bin, GO)
+#undef GO
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
+int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
+                     uint32_t nrows, uint32_t *nnz, hipStream_t s) {
+  if (nrows == 0) return OC_OK;
+  const Tables &t = tables();
+  const size_t lds = 256 + (size_t)(t.dense_sym[dense_bin] / 32 + 2) * 4;
+  KArgs<float> k = kargs<float>(a);
+#define GO(BS)                                                                 \
+  do {                                                                         \
+    OC_TRY(set_lds(k_sym_dense<BS>, lds));                                     \
+    hipLaunchKernelGGL(k_sym_dense<BS>, dim3(nrows), dim3(BS), lds, s, k, rows, \
+                       nnz);                                                   \
+  } while (0)
+  BIN_DISPATCH(dense_bin, GO)
+#undef GO
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
+int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
+                    uint32_t nrows, const NumOut &o, hipStream_t s) {
+  if (nrows == 0) return OC_OK;
+  const Tables &t = tables();
+  const HashLayout L = hash_layout(t.hash_num[bin], t.block[bin] / 64);
+  if (L.bytes > t.lds[bin]) {
+    set_error("numeric hash bin %d layout %u B exceeds %u B", bin, L.bytes, t.lds[bin]);
+    return OC_ERR_INVALID;
+  }
+  const int kb = key_bits_for(a.B.cols);
+  const bool est = o.cap != nullptr;
+#define GO3(BS, T, E)                                                          \
+  do {                                                                         \
+    OC_TRY(set_lds(k_num_hash<BS, T, E>, L.bytes));                            \
+    hipLaunchKernelGGL((k_num_hash<BS, T, E>), dim3(nrows), dim3(BS), L.bytes, \
+                       s, kargs<T>(a), rows, L, kout<T>(o), kb);               \
+  } while (0)
+#define GO(BS)                                            \
+  do {                                                    \
+    if (a.A.vtype == OC_F64) {                            \
+      if (est) GO3(BS, double, true);                     \
+      else GO3(BS, double, false);                        \
+    } else {                                              \
+      if (est) GO3(BS, float, true);                      \
+      else GO3(BS, float, false);                         \
+    }                                                     \
+  } while (0)
+  BIN_DISPATCH(bin, GO)
+#undef GO
+#undef GO3
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
+int launch_num_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
+                     uint32_t nrows, const NumOut &o, hipStream_t s) {
+  if (nrows == 0) return OC_OK;
+  const Tables &t = tables();
+  const uint32_t cap = t.dense_num[dense_bin];
+  const uint32_t nwords = cap / 32 + 2;
+  const size_t lds = 256 + (size_t)nwords * 256 + (size_t)nwords * 4;
+  if (lds > t.lds[dense_bin] + 8) {
+    set_error("dense numeric bin %d needs %zu B", dense_bin, lds);
+    return OC_ERR_INVALID;
+  }
+  const bool est = o.cap != nullptr;
+#define GO3(BS, T, E)                                                           \
+  do {                                                                          \
+    OC_TRY(set_lds(k_num_dense<BS, T, E>, lds));                                \
+    hipLaunchKernelGGL((k_num_dense<BS, T, E>), dim3(nrows), dim3(BS), lds, s,  \
+                       kargs<T>(a), rows, kout<T>(o));                          \
+  } while (0)
+#define GO(BS)                                            \
+  do {                                                    \
+    if (a.A.vtype == OC_F64) {                            \
+      if (est) GO3(BS, double, true);                     \
+      else GO3(BS, double, false);                        \
+    } else {                                              \
+      if (est) GO3(BS, float, true);                      \
+      else GO3(BS, float, false);                         \
+    }                                                     \
+  } while (0)
+  BIN_DISPATCH(dense_bin, GO)
+#undef GO
+#undef GO3
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
+// ============================================================ row sorting
+// Segmented sort of the (col, value) rows of a device CSR in place
+// (kernels/Epilogue.cuh:67-222).  Rows up to kSortLdsMax entries: LDS
+// radix sort of (col, position) with values staged in LDS.  Longer rows:
+// the same stable LSD radix sort streamed through global scratch in
+// 1024-item tiles.
+constexpr int kSortBlock = 1024;
+constexpr uint32_t kSortLdsMax = 6144;
+
+template <typename O, typename T>
+__global__ __launch_bounds__(kSortBlock) void k_sort_rows_lds(
+    const void *rp, uint32_t *__restrict__ cols, T *__restrict__ vals,
+    const uint32_t *__restrict__ rows, uint64_t nrows, int key_bits,
+    uint32_t *long_rows, uint32_t *long_count) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  uint32_t *dsum = reinterpret_cast<uint32_t *>(smem);
+  uint16_t *cnt = reinterpret_cast<uint16_t *>(smem + 1024);
+  uint32_t *k0 = reinterpret_cast<uint32_t *>(smem + 1024 + 16 * 512);
+  uint32_t *k1 = k0 + kSortLdsMax;
+  uint16_t *i0 = reinterpret_cast<uint16_t *>(k1 + kSortLdsMax);
+  uint16_t *i1 = i0 + kSortLdsMax;
+  T *v = reinterpret_cast<T *>(i1 + kSortLdsMax);
+  for (uint64_t t = blockIdx.x; t < nrows; t += gridDim.x) {
+    const uint32_t r = rows ? rows[t] : (uint32_t)t;
+    const uint64_t s = (uint64_t)reinterpret_cast<const O *>(rp)[r];
+    const uint64_t e = (uint64_t)reinterpret_cast<const O *>(rp)[r + 1];
+    const uint64_t n = e - s;
+    if (n > kSortLdsMax) {
+      if (threadIdx.x == 0) long_rows[atomicAdd(long_count, 1u)] = r;
+      continue;
+    }
+    if (n <= 1) continue;
+    for (uint32_t i = threadIdx.x; i < n; i += kSortBlock) {
+      k0[i] = cols[s + i];
+      i0[i] = (uint16_t)i;
+      v[i] = vals[s + i];
+    }
+    __syncthreads();
+    const int par = block_radix_sort(k0, i0, k1, i1, cnt, dsum, (int)n, key_bits);
+    const uint32_t *sk = par ? k1 : k0;
+    const uint16_t *si = par ? i1 : i0;
+    for (uint32_t i = threadIdx.x; i < n; i += kSortBlock) {
+      cols[s + i] = sk[i];
+      vals[s + i] = v[si[i]];
+    }
+    __syncthreads();
+  }
+}
+
+// One block per long row; keys/positions ping-pong in global scratch.
+template <typename O, typename T>
+__global__ __launch_bounds__(kSortBlock) void k_sort_rows_global(
+    const void *rp, uint32_t *__restrict__ cols, T *__restrict__ vals,
+    const uint32_t *__restrict__ long_rows, const uint32_t *long_count,
+    int key_bits, uint8_t *scratch, uint64_t scratch_per_block) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wc[kSortBlock / kWave][256];
+  const int wid = threadIdx.x / kWave, lane = lane_id();
+  const uint32_t nlong = *long_count;
+  uint8_t *my = scratch + (uint64_t)blockIdx.x * scratch_per_block;
+  for (uint32_t t = blockIdx.x; t < nlong; t += gridDim.x) {
+    const uint32_t r = long_rows[t];
+    const uint64_t s = (uint64_t)reinterpret_cast<const O *>(rp)[r];
+    const uint64_t n = (uint64_t)reinterpret_cast<const O *>(rp)[r + 1] - s;
+    uint32_t *ka = reinterpret_cast<uint32_t *>(my);
+    uint32_t *kb = ka + n;
+    uint32_t *pa = kb + n;
+    uint32_t *pb = pa + n;
+    T *vt = reinterpret_cast<T *>(pb + n);
+    for (uint64_t i = threadIdx.x; i < n; i += kSortBlock) {
+      ka[i] = cols[s + i];
+      pa[i] = (uint32_t)i;
+      vt[i] = vals[s + i];
+    }
+    __syncthreads();
+    for (int shift = 0; shift < key_bits; shift += 8) {
+      if (threadIdx.x < 256) base[threadIdx.x] = 0;
+      __syncthreads();
+      for (uint64_t i = threadIdx.x; i < n; i += kSortBlock)
+        atomicAdd(&base[(ka[i] >> shift) & 255u], 1u);
+      __syncthreads();
+      if (threadIdx.x < kWave) {
+        uint32_t t0 = base[4 * lane], t1 = base[4 * lane + 1], t2 = base[4 * lane + 2],
+                 t3 = base[4 * lane + 3];
+        uint32_t sum = t0 + t1 + t2 + t3;
+        uint32_t ex = group_incl_sum(sum, kWave) - sum;
+        base[4 * lane] = ex;
+        base[4 * lane + 1] = ex + t0;
+        base[4 * lane + 2] = ex + t0 + t1;
+        base[4 * lane + 3] = ex + t0 + t1 + t2;
+      }
+      __syncthreads();
+      for (uint64_t t0 = 0; t0 < n; t0 += kSortBlock) {
+        const uint64_t i = t0 + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t key = valid ? ka[i] : 0u;
+        const uint32_t pos0 = valid ? pa[i] : 0u;
+        const uint32_t dg = (key >> shift) & 255u;
+        for (int d = lane; d < 256; d += kWave) wc[wid][d] = 0;
+        __syncthreads();
+        const uint64_t m = peer_mask(dg, valid);
+        if (valid && (m & lanemask_lt()) == 0) wc[wid][dg] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+          uint32_t dst = base[dg] + (uint32_t)__popcll(m & lanemask_lt());
+          for (int w = 0; w < wid; ++w) dst += wc[w][dg];
+          kb[dst] = key;
+          pb[dst] = pos0;
+        }
+        __syncthreads();
+        if (threadIdx.x < 256) {
+          uint32_t add = 0;
+          for (int w = 0; w < kSortBlock / kWave; ++w) add += wc[w][threadIdx.x];
+          base[threadIdx.x] += add;
+        }
+        __syncthreads();
+      }
+      uint32_t *tk = ka;
+      ka = kb;
+      kb = tk;
+      uint32_t *tp = pa;
+      pa = pb;
+      pb = tp;
+      __syncthreads();
+    }
+    for (uint64_t i = threadIdx.x; i < n; i += kSortBlock) {
+      cols[s + i] = ka[i];
+      vals[s + i] = vt[pa[i]];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename O, typename T>
+static int sort_rows_t(oc_csr &M, const uint32_t *rows, uint64_t n,
+                       hipStream_t s) {
+  const int kb = key_bits_for(M.cols);
+  const size_t lds = 1024 + 16 * 512 + (size_t)kSortLdsMax * (4 + 4 + 2 + 2 + sizeof(T));
+  OC_TRY(set_lds(k_sort_rows_lds<O, T>, lds));
+  // host-side: find the longest row to size the global scratch
+  uint32_t *long_info = nullptr;
+  HIP_TRY(hipMallocAsync((void **)&long_info, (n + 64) * 4, s));
+  HIP_TRY(hipMemsetAsync(long_info, 0, 4, s));
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(n, 4096);
+  hipLaunchKernelGGL((k_sort_rows_lds<O, T>), dim3(grid), dim3(kSortBlock), lds, s,
+                     (const void *)M.row_offsets, M.col_ids, (T *)M.values, rows, n,
+                     kb, long_info + 64, long_info);
+  LAUNCH_CHECK();
+  uint32_t nlong = 0;
+  HIP_TRY(hipMemcpyAsync(&nlong, long_info, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (nlong) {
+    // longest listed row
+    std::vector<uint32_t> lr(nlong);
+    HIP_TRY(hipMemcpy(lr.data(), long_info + 64, 4 * nlong, hipMemcpyDeviceToHost));
+    uint64_t maxn = 0;
+    for (uint32_t r : lr) {
+      O o2[2];
+      HIP_TRY(hipMemcpy(o2, (const O *)M.row_offsets + r, 2 * sizeof(O), hipMemcpyDeviceToHost));
+      maxn = std::max<uint64_t>(maxn, (uint64_t)(o2[1] - o2[0]));
+    }
+    const uint32_t g = std::min<uint32_t>(nlong, 64);
+    const uint64_t per = (maxn * (16 + sizeof(T)) + 255) & ~255ull;
+    uint8_t *scr = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&scr, per * g, s));
+    hipLaunchKernelGGL((k_sort_rows_global<O, T>), dim3(g), dim3(kSortBlock), 0, s,
+                       (const void *)M.row_offsets, M.col_ids, (T *)M.values,
+                       (const uint32_t *)(long_info + 64), (const uint32_t *)long_info,
+                       kb, scr, per);
+    LAUNCH_CHECK();
+    HIP_TRY(hipFreeAsync(scr, s));
+  }
+  HIP_TRY(hipFreeAsync(long_info, s));
+  return OC_OK;
+}
+
+int launch_sort_rows(oc_csr &M, const uint32_t *rows, uint64_t n, hipStream_t s) {
+  if (n == 0) return OC_OK;
+  if (M.off_bytes == 8) {
+    if (M.vtype == OC_F64) return sort_rows_t<uint64_t, double>(M, rows, n, s);
+    return sort_rows_t<uint64_t, float>(M, rows, n, s);
+  }
+  if (M.vtype == OC_F64) return sort_rows_t<uint32_t, double>(M, rows, n, s);
+  return sort_rows_t<uint32_t, float>(M, rows, n, s);
+}
+
+}  // namespace oc

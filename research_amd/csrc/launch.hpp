@@ -1,0 +1,143 @@
+// Host-side launchers of the gfx950 kernels (internal to libocean.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "oc_internal.hpp"
+
+namespace oc {
+
+constexpr int kNumBins = 14;   // 7 hash + 7 dense bins (include/Common.h:53)
+constexpr int kSkipBin = 14;   // rows left out of a binned launch
+constexpr int kBinSlots = 15;  // bins 0..13 plus the skip bin
+
+// Device view of a CSR with 32-bit offsets (the A and B inputs).
+struct DevCsr {
+  uint64_t rows, cols, nnz;
+  const uint32_t *rp;
+  const uint32_t *ci;
+  const void *v;
+  int vtype;
+};
+
+// gfx950 tables (oc_tables plus derived launch shapes).
+struct Tables {
+  uint32_t hash_sym[7], dense_sym[7], hash_num[7], dense_num[7];
+  uint32_t block[7], lds[7];
+  uint32_t max_dense_sym_bin;
+  uint32_t win_num_cols;   // numeric window (columns)
+  uint32_t win_sym_cols;   // symbolic window (columns)
+  uint32_t lds_per_cu, num_cu, lds_max_block;
+};
+// Tables for the current device (nominal 160 KB / 256 CUs without a GPU).
+const Tables &tables();
+int tables_init();
+
+// Per-row analysis outputs.
+struct RowInfo {
+  uint32_t *products, *max_b, *avg_b, *left, *right;
+};
+
+int launch_analysis(const DevCsr &A, const DevCsr &B, const RowInfo &ri,
+                    uint32_t *max_a_nnz, uint32_t *max_products,
+                    hipStream_t s);
+size_t scan_temp_bytes(uint64_t n);
+int launch_scan_u32(const uint32_t *in, uint64_t *out, uint64_t n, void *temp,
+                    hipStream_t s);
+int launch_murmur(const uint32_t *cols, uint32_t *out, uint64_t n,
+                  hipStream_t s);
+int launch_hll_build(int log_p, const DevCsr &B, uint8_t *hll, hipStream_t s);
+// mode 0: per-row estimate with expansion and rounding into est[];
+// mode 1: nsamples sampled rows -> sample_rows[], sample_est[] (raw)
+int launch_hll_merge(int log_p, const DevCsr &A, const uint8_t *hll,
+                     double expansion, uint32_t *est, uint32_t nsamples,
+                     uint32_t *sample_rows, int32_t *sample_est,
+                     const uint32_t *products, uint32_t *sample_prod, int mode,
+                     void *temp, hipStream_t s);
+size_t hll_merge_temp_bytes(uint64_t rows);
+
+// Binning.  kind 0 = symbolic rule, 1 = numeric rule.  Rows whose
+// `filter` value is <= skip_below go to the skip bin (filter may be null).
+struct BinParams {
+  int kind;
+  const uint32_t *input;  // products (symbolic) or nnz / estimate (numeric)
+  uint32_t *input_mut;    // numeric: modified for dense-bin-0 rows (est)
+  const uint32_t *left, *right;
+  int use_compaction;
+  double expand_factor, avg_compaction;
+  int expand, modify_dense_est, use_largest;
+  uint32_t hybrid_max;
+  const uint32_t *filter;
+  uint32_t skip_below;
+};
+size_t bin_temp_bytes(uint64_t rows);
+int launch_binning(const BinParams &bp, uint64_t rows, uint8_t *bin_of_row,
+                   uint32_t *bin_rows, uint32_t *bin_offsets, void *temp,
+                   hipStream_t s);
+
+// Copy rows (all rows, or those listed) from a temp layout into C.
+int launch_copy_rows(int vtype, uint64_t nrows, const uint32_t *rows,
+                     const uint32_t *nnz, const uint64_t *src_off,
+                     const uint32_t *src_cols, const void *src_vals,
+                     const uint64_t *dst_off, uint32_t *dst_cols,
+                     void *dst_vals, hipStream_t s);
+
+// Reserve spare space for overflow rows; off[row] = base + reservation.
+int launch_assign_spare(const uint32_t *ovf_rows, uint32_t n,
+                        const uint32_t *products, uint64_t cols, uint64_t base,
+                        unsigned long long *spare_used, uint64_t *off,
+                        hipStream_t s);
+
+// Validation helpers over a device CSR (off_bytes 4 or 8).
+int launch_validate(const oc_csr &M, unsigned long long *bad, hipStream_t s);
+int launch_row_set_hash(const oc_csr &M, uint64_t *hash, hipStream_t s);
+int launch_spmv(const oc_csr &M, const double *x, double *y, hipStream_t s);
+
+// ------------------------------------------------------------ accumulators
+struct AccArgs {
+  DevCsr A, B;
+  const uint32_t *products, *max_b, *left, *right;
+};
+
+// Output description of a numeric launch.
+struct NumOut {
+  const uint64_t *off;     // per-row output offset
+  uint32_t *cols;
+  void *vals;
+  uint32_t *nnz;           // per-row count written (may be null)
+  const uint32_t *cap;     // per-row capacity (est mode) or null
+  uint32_t *ovf_rows;      // est mode: overflow row list
+  uint32_t *ovf_count;
+};
+
+// Symbolic: rows of one bin.  Overflowing rows (estimated-compaction hash
+// tables that fill up) are appended to ovf_rows.
+int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
+                    uint32_t nrows, uint32_t limit, uint32_t *nnz,
+                    uint32_t *ovf_rows, uint32_t *ovf_count, hipStream_t s);
+int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
+                     uint32_t nrows, uint32_t *nnz, hipStream_t s);
+// Windowed kernels take the bin row list (device) and its host-known
+// length; cursor scratch is grid * max_a_nnz u32.
+int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
+                      uint32_t max_a_nnz, uint32_t *nnz, void *scratch,
+                      hipStream_t s);
+size_t window_scratch_bytes(uint32_t max_a_nnz);
+
+int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
+                    uint32_t nrows, const NumOut &o, hipStream_t s);
+int launch_num_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
+                     uint32_t nrows, const NumOut &o, hipStream_t s);
+int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
+                      uint32_t max_a_nnz, const NumOut &o, void *scratch,
+                      hipStream_t s);
+// Ultra-sparse expand-sort-compress for all rows with products <= esc_n.
+int launch_esc(int esc_n, const AccArgs &a, const NumOut &o, hipStream_t s);
+
+// Per-row sort of (col,val) segments of a device CSR.
+int launch_sort_rows(oc_csr &M, const uint32_t *rows, uint64_t n,
+                     hipStream_t s);
+
+}  // namespace oc

@@ -1,0 +1,965 @@
+// SpGEMM orchestrator: workflow decisions, the ultra-sparse, precise and
+// estimation workflows, per-stage timers and the stats JSON.
+//
+// Follows SpGEMM::run and its stage functions (kernels/SpGEMM.cuh:213-1736)
+// with one HIP stream, stream-ordered allocation from the default memory
+// pool (release threshold raised as SpGEMM.cuh:219-223 does) and the gfx950
+// kernels of k_prep.hip / k_accum.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "launch.hpp"
+
+using namespace oc;
+
+#define HIP_TRY(call)                                                         \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                    \
+                hipGetErrorString(e_));                                       \
+      return e_ == hipErrorOutOfMemory ? OC_ERR_NOMEM : OC_ERR_HIP;           \
+    }                                                                         \
+  } while (0)
+
+namespace {
+
+int bits_needed(uint64_t x) {
+  int b = 0;
+  while (x) {
+    ++b;
+    x >>= 1;
+  }
+  return b;
+}
+
+// Stream-ordered allocations released when the run ends.
+struct Arena {
+  hipStream_t s;
+  std::vector<void *> ptrs;
+  explicit Arena(hipStream_t st) : s(st) {}
+  ~Arena() { release(); }
+  template <typename T>
+  int get(T **p, uint64_t count) {
+    size_t bytes = (size_t)std::max<uint64_t>(count, 1) * sizeof(T);
+    bytes = (bytes + 255) & ~(size_t)255;
+    hipError_t e = hipMallocAsync((void **)p, bytes, s);
+    if (e != hipSuccess) {
+      set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+      return OC_ERR_NOMEM;
+    }
+    ptrs.push_back((void *)*p);
+    return OC_OK;
+  }
+  void release() {
+    for (void *p : ptrs) (void)hipFreeAsync(p, s);
+    ptrs.clear();
+  }
+};
+
+// Stream-ordered timestamps; the time between two marks is charged to the
+// later mark's stage.  Resolved once the stream has drained.
+struct Marks {
+  hipStream_t s;
+  std::vector<std::pair<std::string, hipEvent_t>> ev;
+  explicit Marks(hipStream_t st) : s(st) {}
+  ~Marks() {
+    for (auto &p : ev) (void)hipEventDestroy(p.second);
+  }
+  void mark(const char *name) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, s);
+    ev.emplace_back(name, e);
+  }
+  void resolve(std::map<std::string, double> &out, double &total) {
+    total = 0;
+    for (size_t i = 1; i < ev.size(); ++i) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev[i - 1].second, ev[i].second) == hipSuccess) {
+        out[ev[i].first] += ms;
+        total += ms;
+      }
+    }
+  }
+};
+
+template <typename T>
+int readback(T *host, const T *dev, size_t n, hipStream_t s) {
+  HIP_TRY(hipMemcpyAsync(host, dev, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return OC_OK;
+}
+
+struct Decision {
+  int ana1_type = 0;  // 0 ultrasparse, 1 HLL-sampled, 2 fat
+  int spark_kernel_size = 0, spark_need_outlier = 0, spark_outlier_kernel_size = 0;
+  int log_p = 0;
+  double hll_expansion = 0;
+  int nsamples = 0;
+  double comp1 = 0, comp2 = 0;
+  int ana2_type = 0;  // 0 precise, 1 estimation
+  int dense_query_bitmap = 0;
+  double comp_avg = 0, comp_var = 0, safe_comp = 0;
+  int use_est_symbolic = 0;
+  int hash_largest = 0;
+};
+
+}  // namespace
+
+struct oc_spgemm {
+  oc_config cfg;
+  // last run
+  Decision d;
+  uint64_t a_rows = 0, a_cols = 0, a_nnz = 0, b_rows = 0, b_cols = 0, b_nnz = 0;
+  uint64_t c_rows = 0, c_cols = 0, c_nnz = 0;
+  uint64_t total_products = 0;
+  uint32_t max_products = 0, max_a_nnz = 0;
+  uint32_t bin1[kBinSlots + 1] = {0}, bin2[kBinSlots + 1] = {0};
+  bool have_bin1 = false, have_bin2 = false;
+  uint64_t total_est = 0;
+  uint32_t overflow_rows = 0, sym_overflow_rows = 0, precise_recount = 0;
+  int vtype = OC_F64;
+  double total_ms = 0;
+  std::map<std::string, double> stage_ms;
+  // estimation override (learned predictor / forced fallback tests)
+  const uint32_t *est_override = nullptr;
+  int force_workflow = -1;  // -1 none, 0 precise, 1 estimation
+};
+
+namespace {
+
+// ------------------------------------------------------------ decisions
+// SpGEMM.cuh:455-540
+void decide_analysis(const oc_config &c, const oc_spgemm &h, Decision &d) {
+  d = Decision();
+  if (h.a_cols / (uint64_t)c.fat_ratio > h.a_rows &&
+      h.b_rows / (uint64_t)c.fat_ratio > h.b_cols &&
+      h.b_cols < tables().dense_num[5] && c.use_fat_workflow) {
+    d.ana1_type = 2;
+    return;
+  }
+  const double avg = (double)h.total_products / (double)h.a_rows;
+  if (avg <= (double)c.usparse_avg_product_threshold && c.use_ultrasparse_workflow) {
+    static const int sizes[4] = {16, 32, 64, 128};
+    d.ana1_type = 0;
+    for (int k : sizes)
+      if (k >= avg) {
+        d.spark_kernel_size = k;
+        break;
+      }
+    if ((int64_t)h.max_products > d.spark_kernel_size) {
+      d.spark_need_outlier = 1;
+      d.spark_outlier_kernel_size = (int)((h.max_products + 31) / 32 * 32);
+    }
+  } else {
+    const uint64_t ratio = h.a_nnz ? h.total_products / h.a_nnz : 0;
+    d.ana1_type = 1;
+    if (ratio < (uint64_t)c.hll_higher_precision_expansion_threshold) {
+      d.log_p = c.hll_precision_low;
+      d.hll_expansion = c.hll_expansion_low;
+    } else {
+      d.log_p = c.hll_precision_high;
+      d.hll_expansion = c.hll_expansion_high;
+    }
+  }
+}
+
+int num_samples(const oc_config &c, uint64_t rows) {
+  int n = (int)(rows * c.hll_sample_rate);  // SpGEMM.cuh:587-590
+  n = std::max(n, c.hll_sample_min_rows);
+  n = std::min(n, c.hll_sample_max_rows);
+  return n;
+}
+
+// SpGEMM.cuh:610-686 and decideMatrixType (Analysis.cuh:10-29).  Device
+// sums of the reference are taken here in sample order.
+void decide_sampling(const oc_config &c, const oc_spgemm &h,
+                     const std::vector<uint32_t> &prod,
+                     const std::vector<int32_t> &est, Decision &d) {
+  const int n = (int)prod.size();
+  uint64_t sum_est = 0, sum_prod = 0;
+  double sum_comp = 0.0;
+  for (int i = 0; i < n; ++i) {
+    sum_est += (uint64_t)(int64_t)est[i];
+    sum_prod += prod[i];
+    sum_comp += (double)prod[i] / (double)est[i];
+  }
+  d.nsamples = n;
+  d.comp1 = (double)sum_prod / (double)sum_est;
+  d.comp2 = sum_comp / n;
+  // index_t parameters: total_products and A.nnz arrive truncated to 32 bits
+  const uint32_t tp32 = (uint32_t)h.total_products, an32 = (uint32_t)h.a_nnz;
+  int type = 1;
+  if (an32 == 0 || tp32 / an32 < (uint32_t)c.estimation_input_expansion_threshold) type = 0;
+  else if (d.comp1 < c.estimation_output_compaction_threshold) type = 0;
+  else if (d.comp2 < c.estimation_output_compaction_threshold) type = 0;
+  if (!c.use_estimation_workflow) type = 0;
+  d.ana2_type = type;
+  d.dense_query_bitmap = !((d.comp1 < c.dense_query_bitmap_compaction_threshold &&
+                            d.comp2 < c.dense_query_bitmap_compaction_threshold) ||
+                           !c.use_assist_information);
+  if (type == 1) return;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += (double)prod[i] / (double)est[i];
+  d.comp_avg = s / n;
+  double v = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double x = (double)prod[i] / (double)est[i] - d.comp_avg;
+    v += x * x;
+  }
+  d.comp_var = v / (n - 1);
+  d.safe_comp = (d.comp_avg - c.estimate_compaction_target_z_value * std::sqrt(d.comp_var)) / 2;
+  if (d.safe_comp > d.comp_avg * 0.8) d.safe_comp = d.comp_avg * 0.8;
+  d.use_est_symbolic = d.safe_comp >= 1.5 && c.use_assist_information;
+}
+
+// ------------------------------------------------------------- run state
+struct Run {
+  oc_spgemm *h;
+  const oc_config &c;
+  hipStream_t s;
+  Arena ar;
+  Marks mk;
+  DevCsr A, B;
+  RowInfo ri{};
+  uint64_t *psum_prod = nullptr;
+  uint32_t *scal = nullptr;
+  void *scan_tmp = nullptr;
+  uint8_t *hll_b = nullptr;
+  void *win_scratch = nullptr;
+  oc_csr *C;
+  Run(oc_spgemm *hh, hipStream_t st, oc_csr *cc)
+      : h(hh), c(hh->cfg), s(st), ar(st), mk(st), C(cc) {}
+
+  AccArgs acc() const {
+    AccArgs a;
+    a.A = A;
+    a.B = B;
+    a.products = ri.products;
+    a.max_b = ri.max_b;
+    a.left = ri.left;
+    a.right = ri.right;
+    return a;
+  }
+
+  int window_scratch() {
+    if (!win_scratch) {
+      uint8_t *p;
+      OC_TRY(ar.get(&p, window_scratch_bytes(h->max_a_nnz)));
+      win_scratch = p;
+    }
+    return OC_OK;
+  }
+
+  // C: u64 offsets (rows+1) from per-row counts, then cols/values.
+  int alloc_c_from_counts(const uint32_t *nnz) {
+    const uint64_t m = A.rows;
+    std::memset(C, 0, sizeof(*C));
+    C->rows = m;
+    C->cols = B.cols;
+    C->vtype = A.vtype;
+    C->off_bytes = 8;
+    C->on_device = 1;
+    C->owned = 2;  // memory-pool allocation
+    HIP_TRY(hipMallocAsync(&C->row_offsets, (m + 1) * 8, s));
+    OC_TRY(launch_scan_u32(nnz, (uint64_t *)C->row_offsets, m, scan_tmp, s));
+    mk.mark("epilogue_scan");
+    uint64_t nnzc = 0;
+    OC_TRY(readback(&nnzc, (const uint64_t *)C->row_offsets + m, 1, s));
+    C->nnz = nnzc;
+    HIP_TRY(hipMallocAsync((void **)&C->col_ids, std::max<uint64_t>(nnzc, 1) * 4, s));
+    HIP_TRY(hipMallocAsync(&C->values, std::max<uint64_t>(nnzc, 1) * vsize(A.vtype), s));
+    mk.mark("numeric_malloc");
+    return OC_OK;
+  }
+
+  int binning(const BinParams &bp, uint8_t **bor, uint32_t **brows,
+              uint32_t **boff, uint32_t *hoff) {
+    const uint64_t m = A.rows;
+    if (!*bor) {
+      OC_TRY(ar.get(bor, m));
+      OC_TRY(ar.get(brows, m));
+      OC_TRY(ar.get(boff, kBinSlots + 1));
+    }
+    uint8_t *tmp;
+    OC_TRY(ar.get(&tmp, bin_temp_bytes(m)));
+    OC_TRY(launch_binning(bp, m, *bor, *brows, *boff, tmp, s));
+    OC_TRY(readback(hoff, *boff, kBinSlots + 1, s));
+    return OC_OK;
+  }
+
+  // Numeric kernels over numeric bins 1..13 (bin 13 = windowed).
+  int numeric_bins(const uint32_t *brows, const uint32_t *hoff, const NumOut &o,
+                   bool window_bin13) {
+    const AccArgs a = acc();
+    for (int b = 0; b <= 6; ++b) {
+      const uint32_t n = hoff[b + 1] - hoff[b];
+      if (!n) continue;
+      if (b == 0) {
+        set_error("internal: numeric hash bin 0 is not used");
+        return OC_ERR_INVALID;
+      }
+      OC_TRY(launch_num_hash(b, a, brows + hoff[b], n, o, s));
+    }
+    mk.mark("numeric_hash");
+    for (int b = 7; b <= 12; ++b) {
+      const uint32_t n = hoff[b + 1] - hoff[b];
+      if (n) OC_TRY(launch_num_dense(b - 7, a, brows + hoff[b], n, o, s));
+    }
+    mk.mark("numeric_dense");
+    if (window_bin13) {
+      const uint32_t n = hoff[14] - hoff[13];
+      if (n) {
+        OC_TRY(window_scratch());
+        OC_TRY(launch_num_window(a, brows + hoff[13], n, h->max_a_nnz, o, win_scratch, s));
+      }
+      mk.mark("numeric_window");
+    }
+    return OC_OK;
+  }
+
+  int prologue_and_analysis();
+  int sampling();
+  int run_usparse();
+  int run_precise();
+  int run_est();
+};
+
+int Run::prologue_and_analysis() {
+  const uint64_t m = A.rows;
+  // SpGEMM.cuh:360-363
+  if (c.use_hybrid_accumulator && !c.use_indirect_sort) {
+    set_error("Hybrid accumulator is enabled, but indirect sort is disabled. Invalid configuration");
+    return OC_ERR_INVALID;
+  }
+  OC_TRY(ar.get(&ri.products, m));
+  OC_TRY(ar.get(&ri.max_b, m));
+  OC_TRY(ar.get(&ri.avg_b, m));
+  OC_TRY(ar.get(&ri.left, m));
+  OC_TRY(ar.get(&ri.right, m));
+  OC_TRY(ar.get(&psum_prod, m + 1));
+  OC_TRY(ar.get(&scal, 16));
+  uint8_t *st;
+  OC_TRY(ar.get(&st, scan_temp_bytes(m + 1)));
+  scan_tmp = st;
+  // hybrid (largest) numeric hash table rule, SpGEMM.cuh:434-441
+  const uint32_t idx_bits6 = 32 - bits_needed(tables().hash_num[6]);
+  h->d.hash_largest = 0;
+  const bool largest = !((uint32_t)bits_needed(B.cols ? B.cols - 1 : 0) > idx_bits6 ||
+                         !c.use_hybrid_accumulator) &&
+                       B.cols > (uint64_t)c.numeric_hybrid_hash_kernel_row_length_threshold;
+  mk.mark("prologue");
+  OC_TRY(launch_analysis(A, B, ri, scal, scal + 1, s));
+  mk.mark("analysis_product_calc");
+  OC_TRY(launch_scan_u32(ri.products, psum_prod, m, scan_tmp, s));
+  mk.mark("analysis_reduce");
+  uint32_t sc[2];
+  OC_TRY(readback(sc, scal, 2, s));
+  OC_TRY(readback(&h->total_products, psum_prod + m, 1, s));
+  h->max_a_nnz = sc[0];
+  h->max_products = sc[1];
+  mk.mark("analysis_mem_cpy");
+  decide_analysis(c, *h, h->d);
+  h->d.hash_largest = largest;
+  return OC_OK;
+}
+
+int Run::sampling() {
+  Decision &d = h->d;
+  const int P = 1 << d.log_p;
+  OC_TRY(ar.get(&hll_b, B.rows * (uint64_t)P));
+  mk.mark("est_malloc");
+  OC_TRY(launch_hll_build(d.log_p, B, hll_b, s));
+  mk.mark("est_hll_construct");
+  const int ns = num_samples(c, A.rows);
+  uint32_t *srows, *sprod;
+  int32_t *sest;
+  uint8_t *tmp;
+  OC_TRY(ar.get(&srows, ns));
+  OC_TRY(ar.get(&sprod, ns));
+  OC_TRY(ar.get(&sest, ns));
+  OC_TRY(ar.get(&tmp, hll_merge_temp_bytes(ns)));
+  OC_TRY(launch_hll_merge(d.log_p, A, hll_b, 1.0, nullptr, (uint32_t)ns, srows, sest,
+                          ri.products, sprod, 1, tmp, s));
+  std::vector<uint32_t> hp(ns);
+  std::vector<int32_t> he(ns);
+  OC_TRY(readback(hp.data(), sprod, ns, s));
+  OC_TRY(readback(he.data(), sest, ns, s));
+  mk.mark("est_sample");
+  const int hl = d.hash_largest;
+  decide_sampling(c, *h, hp, he, d);
+  d.hash_largest = hl;
+  mk.mark("sampling_var_calc_host");
+  return OC_OK;
+}
+
+int Run::run_usparse() {
+  const Decision &d = h->d;
+  const uint64_t m = A.rows;
+  uint32_t *tc, *nnz;
+  uint8_t *tv;
+  OC_TRY(ar.get(&tc, h->total_products));
+  OC_TRY(ar.get(&tv, h->total_products * vsize(A.vtype)));
+  OC_TRY(ar.get(&nnz, m + 1));
+  mk.mark("numeric_malloc");
+  const int esc_n = std::min(d.spark_kernel_size, 64);
+  NumOut o;
+  std::memset(&o, 0, sizeof(o));
+  o.off = psum_prod;
+  o.cols = tc;
+  o.vals = tv;
+  o.nnz = nnz;
+  OC_TRY(launch_esc(esc_n, acc(), o, s));
+  mk.mark("numeric_usparse_main");
+  if (h->max_products > (uint32_t)esc_n) {
+    uint32_t *ovf;
+    OC_TRY(ar.get(&ovf, m + 4));
+    o.ovf_rows = ovf + 4;
+    o.ovf_count = ovf;
+    HIP_TRY(hipMemsetAsync(ovf, 0, 16, s));
+    BinParams bp;
+    std::memset(&bp, 0, sizeof(bp));
+    bp.kind = 1;
+    bp.input = ri.products;
+    bp.left = ri.left;
+    bp.right = ri.right;
+    bp.expand = 1;
+    bp.expand_factor = c.numeric_expansion_coe;
+    bp.filter = ri.products;
+    bp.skip_below = (uint32_t)esc_n;
+    uint8_t *bor = nullptr;
+    uint32_t *brows = nullptr, *boff = nullptr;
+    uint32_t hoff[kBinSlots + 1];
+    OC_TRY(binning(bp, &bor, &brows, &boff, hoff));
+    std::memcpy(h->bin2, hoff, sizeof(hoff));
+    h->have_bin2 = true;
+    OC_TRY(numeric_bins(brows, hoff, o, true));
+    uint32_t novf = 0;
+    OC_TRY(readback(&novf, ovf, 1, s));
+    if (novf) {
+      set_error("internal: %u ultra-sparse outlier rows overflowed", novf);
+      return OC_ERR_CAPACITY;
+    }
+    mk.mark("numeric_usparse_outlier");
+  }
+  OC_TRY(alloc_c_from_counts(nnz));
+  OC_TRY(launch_copy_rows(A.vtype, m, nullptr, nnz, psum_prod, tc, tv,
+                          (const uint64_t *)C->row_offsets, C->col_ids, C->values, s));
+  mk.mark("epilogue_copy");
+  return OC_OK;
+}
+
+int Run::run_precise() {
+  const Decision &d = h->d;
+  const uint64_t m = A.rows;
+  uint32_t *nnz, *ovf;
+  OC_TRY(ar.get(&nnz, m + 1));
+  OC_TRY(ar.get(&ovf, m + 8));
+  HIP_TRY(hipMemsetAsync(ovf, 0, 32, s));
+  uint32_t *ovf_rows = ovf + 8;
+  BinParams bp;
+  std::memset(&bp, 0, sizeof(bp));
+  bp.kind = 0;
+  bp.input = ri.products;
+  bp.left = ri.left;
+  bp.right = ri.right;
+  bp.use_compaction = d.use_est_symbolic;
+  bp.expand_factor = c.symbolic_expansion_coe;
+  bp.avg_compaction = d.safe_comp;
+  uint8_t *bor = nullptr;
+  uint32_t *brows = nullptr, *boff = nullptr;
+  uint32_t hoff[kBinSlots + 1];
+  OC_TRY(binning(bp, &bor, &brows, &boff, hoff));
+  std::memcpy(h->bin1, hoff, sizeof(hoff));
+  h->have_bin1 = true;
+  mk.mark("symbolic_binning");
+  const AccArgs a = acc();
+  const Tables &t = tables();
+  for (int b = 0; b <= 5; ++b) {
+    const uint32_t n = hoff[b + 1] - hoff[b];
+    if (!n) continue;
+    const uint32_t S = t.hash_sym[b];
+    const uint32_t limit = d.use_est_symbolic ? S - S / 8 : S;
+    OC_TRY(launch_sym_hash(b, a, brows + hoff[b], n, limit, nnz, ovf_rows, ovf, s));
+  }
+  if (hoff[7] > hoff[6]) {
+    OC_TRY(window_scratch());
+    OC_TRY(launch_sym_window(a, brows + hoff[6], hoff[7] - hoff[6], h->max_a_nnz, nnz,
+                             win_scratch, s));
+  }
+  mk.mark("symbolic_hash");
+  for (int b = 7; b <= 13; ++b) {
+    const uint32_t n = hoff[b + 1] - hoff[b];
+    if (!n) continue;
+    if (b == 7) {
+      set_error("internal: dense symbolic bin 0 is not used");
+      return OC_ERR_INVALID;
+    }
+    OC_TRY(launch_sym_dense(b - 7, a, brows + hoff[b], n, nnz, s));
+  }
+  mk.mark("symbolic_dense");
+  if (d.use_est_symbolic) {
+    uint32_t n = 0;
+    OC_TRY(readback(&n, ovf, 1, s));
+    h->sym_overflow_rows = n;
+    if (n) {
+      OC_TRY(window_scratch());
+      OC_TRY(launch_sym_window(a, ovf_rows, n, h->max_a_nnz, nnz, win_scratch, s));
+    }
+    mk.mark("symbolic_overflow");
+  }
+  OC_TRY(alloc_c_from_counts(nnz));
+  // numeric binning on exact counts, SpGEMM.cuh:1247-1283
+  BinParams nb;
+  std::memset(&nb, 0, sizeof(nb));
+  nb.kind = 1;
+  nb.input = nnz;
+  nb.left = ri.left;
+  nb.right = ri.right;
+  nb.expand = 1;
+  nb.expand_factor = c.numeric_expansion_coe;
+  nb.use_largest = d.hash_largest;
+  nb.hybrid_max = (uint32_t)(m / 10);
+  OC_TRY(binning(nb, &bor, &brows, &boff, hoff));
+  std::memcpy(h->bin2, hoff, sizeof(hoff));
+  h->have_bin2 = true;
+  mk.mark("numeric_binning");
+  NumOut o;
+  std::memset(&o, 0, sizeof(o));
+  o.off = (const uint64_t *)C->row_offsets;
+  o.cols = C->col_ids;
+  o.vals = C->values;
+  o.ovf_rows = ovf_rows;
+  o.ovf_count = ovf + 1;
+  OC_TRY(numeric_bins(brows, hoff, o, true));
+  // exact capacities cannot overflow; recount defensively if one did
+  uint32_t n = 0;
+  OC_TRY(readback(&n, ovf + 1, 1, s));
+  h->precise_recount = n;
+  if (n) {
+    OC_TRY(window_scratch());
+    OC_TRY(launch_num_window(a, ovf_rows, n, h->max_a_nnz, o, win_scratch, s));
+  }
+  return OC_OK;
+}
+
+int Run::run_est() {
+  const Decision &d = h->d;
+  const uint64_t m = A.rows;
+  uint32_t *est, *nnz, *ovf;
+  OC_TRY(ar.get(&est, m + 1));
+  OC_TRY(ar.get(&nnz, m + 1));
+  OC_TRY(ar.get(&ovf, m + 8));
+  HIP_TRY(hipMemsetAsync(ovf, 0, 32, s));
+  uint32_t *ovf_rows = ovf + 8;
+  mk.mark("est_malloc");
+  if (h->est_override) {
+    HIP_TRY(hipMemcpyAsync(est, h->est_override, m * 4, hipMemcpyDeviceToDevice, s));
+  } else {
+    uint8_t *tmp;
+    OC_TRY(ar.get(&tmp, hll_merge_temp_bytes(m)));
+    OC_TRY(launch_hll_merge(d.log_p, A, hll_b, d.hll_expansion, est, 0, nullptr, nullptr,
+                            nullptr, nullptr, 0, tmp, s));
+  }
+  mk.mark("est_hll_merge");
+  BinParams bp;
+  std::memset(&bp, 0, sizeof(bp));
+  bp.kind = 1;
+  bp.input = est;
+  bp.input_mut = est;
+  bp.left = ri.left;
+  bp.right = ri.right;
+  bp.modify_dense_est = 1;
+  bp.use_largest = d.hash_largest;
+  bp.hybrid_max = (uint32_t)(m / 10);
+  uint8_t *bor = nullptr;
+  uint32_t *brows = nullptr, *boff = nullptr;
+  uint32_t hoff[kBinSlots + 1];
+  OC_TRY(binning(bp, &bor, &brows, &boff, hoff));
+  std::memcpy(h->bin2, hoff, sizeof(hoff));
+  h->have_bin2 = true;
+  uint64_t *psum_est;
+  OC_TRY(ar.get(&psum_est, m + 1));
+  OC_TRY(launch_scan_u32(est, psum_est, m, scan_tmp, s));
+  OC_TRY(readback(&h->total_est, psum_est + m, 1, s));
+  mk.mark("numeric_binning");
+  const double coe = c.hll_numeric_malloc_expansion_coe;
+  const uint64_t cap_total = std::max<uint64_t>((uint64_t)((double)h->total_est * coe), h->total_est);
+  uint32_t *tc;
+  uint8_t *tv;
+  OC_TRY(ar.get(&tc, cap_total));
+  OC_TRY(ar.get(&tv, cap_total * vsize(A.vtype)));
+  mk.mark("numeric_malloc");
+  NumOut o;
+  std::memset(&o, 0, sizeof(o));
+  o.off = psum_est;
+  o.cols = tc;
+  o.vals = tv;
+  o.nnz = nnz;
+  o.cap = est;
+  o.ovf_rows = ovf_rows;
+  o.ovf_count = ovf;
+  OC_TRY(numeric_bins(brows, hoff, o, false));
+  // overflow rows plus the windowed-bin rows go to the spare region
+  uint32_t n1 = 0;
+  OC_TRY(readback(&n1, ovf, 1, s));
+  const uint32_t n13 = hoff[14] - hoff[13];
+  if (n13)
+    HIP_TRY(hipMemcpyAsync(ovf_rows + n1, brows + hoff[13], (size_t)n13 * 4,
+                           hipMemcpyDeviceToDevice, s));
+  const uint32_t novf = n1 + n13;
+  h->overflow_rows = n1;
+  uint64_t *src_off;
+  OC_TRY(ar.get(&src_off, m + 1));
+  HIP_TRY(hipMemcpyAsync(src_off, psum_est, (m + 1) * 8, hipMemcpyDeviceToDevice, s));
+  if (novf) {
+    unsigned long long *used;
+    OC_TRY(ar.get(&used, 1));
+    HIP_TRY(hipMemsetAsync(used, 0, 8, s));
+    OC_TRY(launch_assign_spare(ovf_rows, novf, ri.products, B.cols, h->total_est, used,
+                               src_off, s));
+    unsigned long long hu = 0;
+    OC_TRY(readback(&hu, used, 1, s));
+    if (h->total_est + hu > cap_total) {
+      set_error("Not enough overflow buffer allocated. Try to increase "
+                "hll_numeric_malloc_expansion_coe (default at 1.3).");
+      return OC_ERR_CAPACITY;
+    }
+    NumOut o2 = o;
+    o2.off = src_off;
+    o2.cap = nullptr;
+    OC_TRY(window_scratch());
+    OC_TRY(launch_num_window(acc(), ovf_rows, novf, h->max_a_nnz, o2, win_scratch, s));
+  }
+  mk.mark("numeric_overflow");
+  OC_TRY(alloc_c_from_counts(nnz));
+  OC_TRY(launch_copy_rows(A.vtype, m, nullptr, nnz, src_off, tc, tv,
+                          (const uint64_t *)C->row_offsets, C->col_ids, C->values, s));
+  mk.mark("epilogue_copy");
+  return OC_OK;
+}
+
+// ------------------------------------------------------------------ JSON
+struct Json {
+  std::ostringstream o;
+  bool first = true;
+  void key(const char *k) {
+    if (!first) o << ",";
+    first = false;
+    o << "\"" << k << "\":";
+  }
+  void num(const char *k, double v) {
+    key(k);
+    if (std::isfinite(v)) {
+      o.precision(17);
+      o << v;
+    } else {
+      o << "null";
+    }
+  }
+  void u64(const char *k, uint64_t v) {
+    key(k);
+    o << v;
+  }
+  void boolean(const char *k, bool v) {
+    key(k);
+    o << (v ? "true" : "false");
+  }
+  void open(const char *k) {
+    key(k);
+    o << "{";
+    first = true;
+  }
+  void close() {
+    o << "}";
+    first = false;
+  }
+  void arr(const char *k, const uint32_t *v, int n) {
+    key(k);
+    o << "[";
+    for (int i = 0; i < n; ++i) o << (i ? "," : "") << v[i];
+    o << "]";
+  }
+};
+
+double stage(const oc_spgemm *h, const char *k) {
+  auto it = h->stage_ms.find(k);
+  return it == h->stage_ms.end() ? 0.0 : it->second;
+}
+
+std::string stats_json(const oc_spgemm *h) {
+  Json j;
+  j.o << "{";
+  j.open("A");
+  j.u64("rows", h->a_rows);
+  j.u64("cols", h->a_cols);
+  j.u64("nnz", h->a_nnz);
+  j.close();
+  j.open("B");
+  j.u64("rows", h->b_rows);
+  j.u64("cols", h->b_cols);
+  j.u64("nnz", h->b_nnz);
+  j.close();
+  j.open("C");
+  j.u64("rows", h->c_rows);
+  j.u64("cols", h->c_cols);
+  j.u64("nnz", h->c_nnz);
+  j.close();
+  j.open("stats");
+  j.u64("intermediate_product", h->total_products);
+  j.u64("intermediate_product_max", h->max_products);
+  j.num("intermediate_product_avg", (double)h->total_products / (double)h->a_rows);
+  j.num("input_expansion", (double)h->total_products / (double)h->a_nnz);
+  j.num("output_compression", (double)h->total_products / (double)h->c_nnz);
+  j.close();
+  const Decision &d = h->d;
+  int workflow = 0;  // JSON ids of SpGEMM.cuh:1807-1817
+  if (d.ana1_type == 0) workflow = 2;
+  else if (d.ana1_type == 2) workflow = 3;
+  else if (d.ana2_type == 0) workflow = 0;
+  else workflow = 1;
+  j.open("decision");
+  j.u64("workflow", workflow);
+  if (workflow <= 0) j.boolean("estimated_symbolic", d.use_est_symbolic);
+  if (workflow <= 1) {
+    j.boolean("hash_largest", d.hash_largest);
+    j.boolean("dense_bitmap_query", d.dense_query_bitmap);
+  }
+  if (workflow == 3) j.u64("fat_runs_precise_kernels", 1);
+  j.close();
+  if (workflow <= 1) {
+    j.open("sampling");
+    j.num("compaction_1", d.comp1);
+    j.num("compaction_2", d.comp2);
+    if (workflow <= 0) {
+      j.num("compaction_2_var", d.comp_var);
+      j.num("safe_estimated_compaction", d.safe_comp);
+    }
+    j.close();
+  }
+  if (h->have_bin1) j.arr("binning_1", h->bin1, kBinSlots + 1);
+  if (h->have_bin2) j.arr("binning_2", h->bin2, kBinSlots + 1);
+  if (workflow == 1) {
+    j.open("estimation");
+    j.u64("overflow_rows", h->overflow_rows);
+    j.u64("log2_precision", d.log_p);
+    j.u64("total_est", h->total_est);
+    j.num("expansion", h->cfg.hll_numeric_malloc_expansion_coe);
+    j.close();
+  }
+  if (workflow == 2) {
+    j.open("ultra_sparse");
+    j.u64("kernel_size", d.spark_kernel_size);
+    j.u64("outlier_size", d.spark_outlier_kernel_size);
+    j.close();
+  }
+  if (h->cfg.track_stage_times) {
+    j.open("timing");
+    j.num("prologue", stage(h, "prologue"));
+    j.open("analysis");
+    j.num("product_calc", stage(h, "analysis_product_calc"));
+    j.num("reduce", stage(h, "analysis_reduce"));
+    j.num("mem_cpy", stage(h, "analysis_mem_cpy"));
+    j.close();
+    j.open("sampling");
+    j.num("var_calc_host", stage(h, "sampling_var_calc_host"));
+    j.close();
+    j.open("estimation");
+    j.num("sampling", stage(h, "est_sample"));
+    j.num("malloc", stage(h, "est_malloc"));
+    j.num("hll_construct", stage(h, "est_hll_construct"));
+    j.num("hll_merge", stage(h, "est_hll_merge"));
+    j.num("binning", 0.0);
+    j.close();
+    j.open("symbolic");
+    j.num("binning", stage(h, "symbolic_binning"));
+    j.num("hash", stage(h, "symbolic_hash"));
+    j.num("dense", stage(h, "symbolic_dense"));
+    j.num("overflow", stage(h, "symbolic_overflow"));
+    j.close();
+    j.open("numeric");
+    j.num("malloc", stage(h, "numeric_malloc"));
+    j.num("binning", stage(h, "numeric_binning"));
+    j.num("hash", stage(h, "numeric_hash"));
+    j.num("usparse_outlier", stage(h, "numeric_usparse_outlier"));
+    j.num("usparse_main", stage(h, "numeric_usparse_main"));
+    j.num("dense", stage(h, "numeric_dense"));
+    j.num("window", stage(h, "numeric_window"));
+    j.num("overflow", stage(h, "numeric_overflow"));
+    j.num("fat", 0.0);
+    j.close();
+    j.open("epilogue");
+    j.num("scan", stage(h, "epilogue_scan"));
+    j.num("sort", 0.0);
+    j.num("copy", stage(h, "epilogue_copy"));
+    j.close();
+    j.close();
+  }
+  j.num("total_time", h->total_ms);
+  j.num("gflops", (double)(h->total_products * 2) / h->total_ms / 1e6);
+  // HBM roofline of the whole product (gather model)
+  const double s = (double)vsize(h->vtype);
+  const double oc_bytes = h->c_nnz >= 0xFFFFFFFFull ? 8.0 : 4.0;
+  const double bytes = 4.0 * (h->a_rows + 1) + (4 + s) * h->a_nnz +
+                       (4 + s) * (double)h->total_products + oc_bytes * (h->a_rows + 1) +
+                       (4 + s) * (double)h->c_nnz;
+  j.open("roofline");
+  j.num("bytes_alg", bytes);
+  j.num("achieved_gbps", bytes / (h->total_ms * 1e-3) / 1e9);
+  j.num("fraction_of_8tbps", bytes / (h->total_ms * 1e-3) / 8.0e12);
+  j.close();
+  j.o << "}";
+  return j.o.str();
+}
+
+int validate_inputs(const oc_csr *A, const oc_csr *B) {
+  OC_REQUIRE(A && B, "null matrix");
+  OC_REQUIRE(A->on_device && B->on_device, "A and B must be device matrices");
+  OC_REQUIRE(A->off_bytes == 4 && B->off_bytes == 4,
+             "A and B need 32-bit row offsets (off_bytes 4)");
+  OC_REQUIRE(A->vtype == B->vtype, "A and B must have the same value type");
+  OC_REQUIRE(A->vtype == OC_F64 || A->vtype == OC_F32, "bad value type");
+  OC_REQUIRE(A->cols == B->rows, "Incompatible matrix dimensions! (A.cols %llu != B.rows %llu)",
+             (unsigned long long)A->cols, (unsigned long long)B->rows);
+  OC_REQUIRE(A->rows < (1ull << 32) && B->cols <= (1ull << 32),
+             "row and column counts must fit 32-bit indices");
+  return OC_OK;
+}
+
+int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
+                hipStream_t s) {
+  OC_TRY(validate_inputs(dA, dB));
+  OC_REQUIRE(dC != nullptr, "null output matrix");
+  OC_TRY(tables_init());
+  h->a_rows = dA->rows;
+  h->a_cols = dA->cols;
+  h->a_nnz = dA->nnz;
+  h->b_rows = dB->rows;
+  h->b_cols = dB->cols;
+  h->b_nnz = dB->nnz;
+  h->vtype = dA->vtype;
+  h->have_bin1 = h->have_bin2 = false;
+  h->total_est = 0;
+  h->overflow_rows = h->sym_overflow_rows = h->precise_recount = 0;
+  h->stage_ms.clear();
+  std::memset(dC, 0, sizeof(*dC));
+  Run r(h, s, dC);
+  r.A = DevCsr{dA->rows, dA->cols, dA->nnz, (const uint32_t *)dA->row_offsets,
+               dA->col_ids, dA->values, dA->vtype};
+  r.B = DevCsr{dB->rows, dB->cols, dB->nnz, (const uint32_t *)dB->row_offsets,
+               dB->col_ids, dB->values, dB->vtype};
+  r.mk.mark("start");
+  int rc = OC_OK;
+  if (dA->rows == 0) {
+    dC->rows = 0;
+    dC->cols = dB->cols;
+    dC->vtype = dA->vtype;
+    dC->off_bytes = 8;
+    dC->on_device = 1;
+    dC->owned = 2;
+    HIP_TRY(hipMallocAsync(&dC->row_offsets, 8, s));
+    HIP_TRY(hipMemsetAsync(dC->row_offsets, 0, 8, s));
+    HIP_TRY(hipMallocAsync((void **)&dC->col_ids, 4, s));
+    HIP_TRY(hipMallocAsync(&dC->values, 8, s));
+    h->total_products = 0;
+    h->d = Decision();
+  } else {
+    rc = r.prologue_and_analysis();
+    if (rc == OC_OK && h->d.ana1_type == 1) rc = r.sampling();
+    if (rc == OC_OK && h->force_workflow >= 0 && h->d.ana1_type == 1)
+      h->d.ana2_type = h->force_workflow;
+    if (rc == OC_OK) {
+      const Decision &d = h->d;
+      if (d.ana1_type == 0) rc = r.run_usparse();
+      else if (d.ana1_type == 2 || d.ana2_type == 0) rc = r.run_precise();
+      else rc = r.run_est();
+    }
+  }
+  r.mk.mark("end");
+  hipError_t e = hipStreamSynchronize(s);
+  if (rc == OC_OK && e != hipSuccess) {
+    set_error("stream error: %s", hipGetErrorString(e));
+    rc = OC_ERR_HIP;
+  }
+  r.mk.resolve(h->stage_ms, h->total_ms);
+  r.ar.release();
+  if (rc != OC_OK) {
+    oc_csr_free(dC);
+    return rc;
+  }
+  h->c_rows = dC->rows;
+  h->c_cols = dC->cols;
+  h->c_nnz = dC->nnz;
+  return OC_OK;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" oc_spgemm *oc_spgemm_create(const oc_config *cfg) {
+  oc_spgemm *h = new (std::nothrow) oc_spgemm();
+  if (!h) {
+    set_error("out of host memory");
+    return nullptr;
+  }
+  if (cfg) h->cfg = *cfg;
+  else oc_config_default(&h->cfg);
+  // Keep freed pool memory for reuse across runs (SpGEMM.cuh:219-223).
+  int dev = 0;
+  hipMemPool_t pool;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t thr = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+  }
+  (void)hipGetLastError();
+  return h;
+}
+
+extern "C" void oc_spgemm_destroy(oc_spgemm *h) { delete h; }
+
+extern "C" int oc_spgemm_run(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB,
+                             oc_csr *dC, void *stream) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  return spgemm_impl(h, dA, dB, dC, (hipStream_t)stream);
+}
+
+extern "C" int oc_spgemm_stats_json(const oc_spgemm *h, char *buf, size_t cap,
+                                    size_t *needed) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  const std::string js = stats_json(h);
+  if (needed) *needed = js.size() + 1;
+  if (buf && cap) {
+    size_t n = std::min(cap - 1, js.size());
+    std::memcpy(buf, js.data(), n);
+    buf[n] = '\0';
+  }
+  return OC_OK;
+}
+
+extern "C" int oc_spgemm_set_workflow(oc_spgemm *h, int workflow) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  OC_REQUIRE(workflow >= -1 && workflow <= 1, "workflow must be -1, 0 or 1");
+  h->force_workflow = workflow;
+  return OC_OK;
+}
+
+extern "C" int oc_spgemm_set_estimates(oc_spgemm *h, const uint32_t *d_est) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  h->est_override = d_est;
+  return OC_OK;
+}
+
+extern "C" double oc_spgemm_timer_ms(const oc_spgemm *h, const char *name) {
+  if (!h || !name) return -1.0;
+  if (!std::strcmp(name, "total")) return h->total_ms;
+  auto it = h->stage_ms.find(name);
+  return it == h->stage_ms.end() ? -1.0 : it->second;
+}

@@ -1,0 +1,79 @@
+"""End-to-end GPU parity of C = A*B against the FP64 oracle (structure
+bit-exact, values within the condition-normalised tolerance)."""
+import numpy as np
+import pytest
+
+from helpers import check_product, log
+
+pytestmark = pytest.mark.gpu
+
+TOL64, TOL32 = 1e-12, 1e-5
+
+
+def _run(gpu, A, B, cfg=None, workflow=None):
+    dA, dB = A.upload(), B.upload()
+    sp = gpu.SpGEMM(cfg or gpu.Config())
+    if workflow is not None:
+        sp.set_workflow(workflow)
+    dC = sp.run(dA, dB)
+    assert dC.validate() == 0
+    C = dC.download()
+    return C, sp.get_stats()
+
+
+def test_laplacian_ultrasparse_bit_exact(gpu):
+    log("case: laplacian 256x256 f64 seed0 (ESC)")
+    A = gpu.CSR.laplacian2d(256, 256, seed=0)
+    C, st = _run(gpu, A, A)
+    assert st["decision"]["workflow"] == 2
+    assert C.nnz == 846852 and st["stats"]["intermediate_product"] == 1629192
+    worst, plain = check_product(A, A, C, TOL64, "laplacian", plain_exact=True)
+    log(f"laplacian: normalised {worst:.2e} plain {plain:.2e}")
+
+
+@pytest.mark.parametrize("log_n,deg", [(10, 8), (12, 16), (14, 16)])
+def test_rmat_default_workflow(gpu, log_n, deg):
+    log(f"case: rmat 2^{log_n} deg {deg} f64 (auto workflow)")
+    A = gpu.CSR.rmat(log_n, deg << log_n, seed=1)
+    C, st = _run(gpu, A, A)
+    worst, plain = check_product(A, A, C, TOL64, f"rmat{log_n}")
+    log(f"rmat {log_n}: workflow {st['decision']['workflow']} err {worst:.2e} plain {plain:.2e}")
+
+
+@pytest.mark.parametrize("workflow", [0, 1])
+def test_rmat_forced_workflows(gpu, workflow):
+    log(f"case: rmat 2^13 deg 16 f64 forced workflow {workflow}")
+    A = gpu.CSR.rmat(13, 16 << 13, seed=3)
+    cfg = gpu.Config(use_ultrasparse_workflow=0)
+    C, st = _run(gpu, A, A, cfg, workflow)
+    assert st["decision"]["workflow"] == workflow
+    worst, plain = check_product(A, A, C, TOL64, f"rmat13 wf{workflow}")
+    log(f"forced {workflow}: err {worst:.2e}, overflow rows {st.get('estimation', {}).get('overflow_rows')}")
+
+
+def test_rectangular_f32(gpu):
+    log("case: rectangular 2^14 x 2^17 poisson(8) f32, A*A^T")
+    A = gpu.CSR.rect_band(1 << 14, 1 << 17, 8.0, 0.3, 4096, 8, vtype=1, value_dist=1, seed=2)
+    B = A.transpose()
+    C, st = _run(gpu, A, B)
+    worst, plain = check_product(A, B, C, TOL32, "rect")
+    log(f"rect: workflow {st['decision']['workflow']} err {worst:.2e}")
+
+
+@pytest.mark.parametrize("kind", ["er", "banded"])
+def test_small_types_f32(gpu, kind):
+    log(f"case: {kind} f32")
+    if kind == "er":
+        A = gpu.CSR.erdos_renyi(3000, 12.0, vtype=1, seed=4)
+    else:
+        A = gpu.CSR.banded(5000, 33, vtype=1, seed=4)
+    C, st = _run(gpu, A, A)
+    check_product(A, A, C, TOL32, kind)
+
+
+def test_precise_dense_and_hash_bins(gpu):
+    log("case: precise on ER 4000 deg 40 (hash + dense bins)")
+    A = gpu.CSR.erdos_renyi(4000, 40.0, seed=7)
+    C, st = _run(gpu, A, A, gpu.Config(use_ultrasparse_workflow=0), 0)
+    check_product(A, A, C, TOL64, "er-precise")
+    log(f"bins: {st.get('binning_1')} / {st.get('binning_2')}")
