@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""SpGEMM benchmark on one MI355X: C = A*A with A the R-MAT n=2^20 matrix
+(16 nnz/row on average, (a,b,c,d) = (.57,.19,.19,.05), duplicates merged,
+FP64 values uniform[-1,1) from seed 1, int32 indices).
+
+A "step" is one full C = A*A through libocean.so (analysis, HLL sampling,
+binning, symbolic, numeric, output allocation; the output is freed between
+steps).  A and B are resident in HBM before the timed region.  Prints one
+JSON line:
+  value     GFLOP/s = 2 * intermediate products / step time
+  roofline  dominant kernel family, its algorithmic bytes per launch over its
+            average launch time (HIP events recorded on the launch stream by
+            the library during the timed steps) against 8 TB/s
+  cpu_baseline  FP64 row-wise Gustavson (the oracle's, OpenMP) on a bounded
+            sample of the same rows, on this box's host cores
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SpGEMM GFLOP/s + % HBM roofline; learned+batched vs plain HLL path speedup"
+PEAK_HBM_GBPS = 8000.0
+
+
+def workload(name: str, ra):
+    if name == "rmat20":
+        A = ra.CSR.rmat(20, 16 << 20, 0.57, 0.19, 0.19, vtype=0, seed=1)
+        desc = "R-MAT n=2^20, 16 nnz/row avg, (a,b,c,d)=(.57,.19,.19,.05), dedup, FP64 U[-1,1) seed 1, C=A*A"
+        return A, A, desc, "f64"
+    if name == "laplacian":
+        A = ra.CSR.laplacian2d(256, 256, vtype=0, seed=0)
+        return A, A, "2D 5-point Laplacian 256x256, FP64 U[-1,1) seed 0, C=A*A", "f64"
+    if name == "rect":
+        A = ra.CSR.rect_band(1 << 18, 1 << 21, 8.0, 0.3, 4096, 8, vtype=1, value_dist=1, seed=2)
+        return A, A.transpose(), "2^18 x 2^21 Poisson(8)/row, 30% in row*8+-4096, FP32 N(0,1) seed 2, C=A*A^T", "f32"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(A, B, budget_s: float, nthreads: int):
+    """Oracle Gustavson on row panels spread over the matrix."""
+    from oracle import oracle as ora
+    m = A.rows
+    npan = 64
+    pan = max(1, m // (npan * 8))  # 1/8 of the rows in 64 panels
+    starts = [int(i * (m // npan)) for i in range(npan)]
+    rl = np.diff(B.row_offsets.astype(np.int64))
+    ro = A.row_offsets.astype(np.int64)
+    cs = np.concatenate(([0], np.cumsum(rl[A.col_ids.astype(np.int64)])))
+    prod_row = cs[ro[1:]] - cs[ro[:-1]]
+    done_prod, done_rows, t_used = 0, 0, 0.0
+    for s in starts:
+        e = min(m, s + pan)
+        t0 = time.perf_counter()
+        ora.numeric(A, B, s, e, compensated=0, with_cond=False, nthreads=nthreads)
+        t_used += time.perf_counter() - t0
+        done_prod += int(prod_row[s:e].sum())
+        done_rows += e - s
+        if t_used > budget_s:
+            break
+    gflops = 2.0 * done_prod / t_used / 1e9
+    return {
+        "value": round(gflops, 3),
+        "unit": "GFLOP/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"{done_rows} rows in {pan}-row panels spread over the matrix "
+                  f"({done_prod} products, {t_used:.1f} s); symbolic + numeric Gustavson, "
+                  f"dense accumulator + touched list + per-row sort, FP64",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "laplacian", "rect"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(row panels of A would shard across GPUs; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import research_amd as ra
+    from research_amd import ocean
+
+    t0 = time.perf_counter()
+    A, B, desc, dtype = workload(args.workload, ra)
+    print(f"[bench] workload built in {time.perf_counter() - t0:.1f} s: {desc}", file=sys.stderr, flush=True)
+    dA = A.upload()
+    dB = dA if B is A else B.upload()
+    sp = ra.SpGEMM(ra.Config(track_stage_times=1))
+
+    for i in range(args.warmup):
+        C = sp.run(dA, dB)
+        print(f"[bench] warmup {i}: {sp.timer_ms('total'):.1f} ms, nnz(C)={C.nnz}", file=sys.stderr, flush=True)
+        C.free()
+    ocean.lib.oc_device_synchronize()
+
+    fam_ms, fam_bytes, fam_info = {}, {}, {}
+    step_ms = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        C = sp.run(dA, dB)
+        step_ms.append(sp.timer_ms("total"))
+        st = sp.get_stats()
+        for k, v in st.get("kernels", {}).items():
+            fam_ms[k] = fam_ms.get(k, 0.0) + v["ms"]
+            fam_bytes[k] = v["bytes_alg"]
+            fam_info[k] = v
+        nnz_c = C.nnz
+        print(f"[bench] step {i}: {step_ms[-1]:.1f} ms; stages " +
+              ", ".join(f"{k}={v:.1f}" for k, v in st.get("timing_flat", {}).items()),
+              file=sys.stderr, flush=True)
+        if i < args.steps - 1:
+            C.free()
+    ocean.lib.oc_device_synchronize()
+    t_total = time.perf_counter() - t_start
+    bad = C.validate()
+    C.free()
+    ms_per_step = t_total * 1e3 / args.steps
+    products = st["stats"]["intermediate_product"]
+    value = 2.0 * products / (ms_per_step * 1e-3) / 1e9
+
+    dom = max(fam_ms, key=lambda k: fam_ms[k]) if fam_ms else None
+    roof = None
+    if dom:
+        avg_ms = fam_ms[dom] / args.steps
+        ach = fam_bytes[dom] / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": PEAK_HBM_GBPS,
+                "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBPS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 3), "bytes_alg_per_launch": fam_bytes[dom]}
+    whole = st.get("roofline", {})
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {"workload": desc, "products": products, "nnz_c": st["C"]["nnz"],
+                   "workflow": st["decision"]["workflow"], "parallelism": "single GPU"},
+        "roofline": roof,
+        "whole_product_roofline": {
+            "bytes_alg": whole.get("bytes_alg"),
+            "achieved_gbps": round(whole.get("bytes_alg", 0) / (ms_per_step * 1e-3) / 1e9, 2),
+            "frac": round(whole.get("bytes_alg", 0) / (ms_per_step * 1e-3) / 8e12, 4)},
+        "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in fam_ms.items()},
+        "library_ms_per_step": round(float(np.mean(step_ms)), 3),
+        "structure_invalid_rows": bad,
+    }
+    if not args.no_cpu_baseline:
+        nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+        out["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds, nth)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

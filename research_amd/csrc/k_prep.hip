@@ -844,6 +844,47 @@ int launch_assign_spare(const uint32_t *ovf_rows, uint32_t n,
   return OC_OK;
 }
 
+// --------------------------------------------------- work accounting
+// Sums over a row list (or all rows): A entries, products and output
+// entries (from u64 offsets or a count array); out[0..3] += {rows, a_nnz,
+// products, out_nnz}.
+__global__ void k_row_sums(const uint32_t *__restrict__ rows, uint32_t n,
+                           const uint32_t *__restrict__ a_rp,
+                           const uint32_t *__restrict__ products,
+                           const uint64_t *__restrict__ c_rp,
+                           const uint32_t *__restrict__ cnt,
+                           unsigned long long *out) {
+  unsigned long long an = 0, pr = 0, nz = 0, k = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t r = rows ? rows[i] : i;
+    an += a_rp[r + 1] - a_rp[r];
+    pr += products[r];
+    nz += c_rp ? c_rp[r + 1] - c_rp[r] : (cnt ? cnt[r] : 0u);
+    ++k;
+  }
+  an = group_sum(an, kWave);
+  pr = group_sum(pr, kWave);
+  nz = group_sum(nz, kWave);
+  k = group_sum(k, kWave);
+  if (lane_id() == 0) {
+    atomicAdd(&out[0], k);
+    atomicAdd(&out[1], an);
+    atomicAdd(&out[2], pr);
+    atomicAdd(&out[3], nz);
+  }
+}
+
+int launch_row_sums(const uint32_t *rows, uint32_t n, const uint32_t *a_rp,
+                    const uint32_t *products, const uint64_t *c_rp,
+                    const uint32_t *cnt, unsigned long long *out, hipStream_t s) {
+  if (n == 0) return OC_OK;
+  hipLaunchKernelGGL(k_row_sums, dim3(std::min<uint32_t>(cdiv(n, 256), 1024)), dim3(256),
+                     0, s, rows, n, a_rp, products, c_rp, cnt, out);
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
 // ------------------------------------------------------------ validation
 template <typename O>
 __device__ __forceinline__ uint64_t off_at(const void *p, uint64_t i) {

@@ -90,6 +90,12 @@ int launch_assign_spare(const uint32_t *ovf_rows, uint32_t n,
                         unsigned long long *spare_used, uint64_t *off,
                         hipStream_t s);
 
+// Work accounting over a row list: out[0..3] += {rows, a_nnz, products,
+// out_nnz} (out_nnz from u64 offsets c_rp, else from cnt).
+int launch_row_sums(const uint32_t *rows, uint32_t n, const uint32_t *a_rp,
+                    const uint32_t *products, const uint64_t *c_rp,
+                    const uint32_t *cnt, unsigned long long *out, hipStream_t s);
+
 // Validation helpers over a device CSR (off_bytes 4 or 8).
 int launch_validate(const oc_csr &M, unsigned long long *bad, hipStream_t s);
 int launch_row_set_hash(const oc_csr &M, uint64_t *hash, hipStream_t s);

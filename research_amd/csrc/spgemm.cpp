@@ -129,6 +129,12 @@ struct oc_spgemm {
   int vtype = OC_F64;
   double total_ms = 0;
   std::map<std::string, double> stage_ms;
+  struct FamStat {
+    std::string name, stage;
+    bool numeric;
+    unsigned long long rows, a_nnz, products, nnz;
+  };
+  std::vector<FamStat> fams;
   // estimation override (learned predictor / forced fallback tests)
   const uint32_t *est_override = nullptr;
   int force_workflow = -1;  // -1 none, 0 precise, 1 estimation
@@ -236,6 +242,17 @@ struct Run {
   uint8_t *hll_b = nullptr;
   void *win_scratch = nullptr;
   oc_csr *C;
+  struct Fam {
+    const char *name, *stage;
+    bool numeric;
+    const uint32_t *rows;  // null: all rows
+    uint32_t n;
+  };
+  std::vector<Fam> fams;
+  void fam(const char *name, const char *stage, bool numeric, const uint32_t *rows,
+           uint32_t n) {
+    if (c.track_stage_times && n) fams.push_back(Fam{name, stage, numeric, rows, n});
+  }
   Run(oc_spgemm *hh, hipStream_t st, oc_csr *cc)
       : h(hh), c(hh->cfg), s(st), ar(st), mk(st), C(cc) {}
 
@@ -309,11 +326,13 @@ struct Run {
       }
       OC_TRY(launch_num_hash(b, a, brows + hoff[b], n, o, s));
     }
+    fam("num_hash", "numeric_hash", true, brows + hoff[1], hoff[7] - hoff[1]);
     mk.mark("numeric_hash");
     for (int b = 7; b <= 12; ++b) {
       const uint32_t n = hoff[b + 1] - hoff[b];
       if (n) OC_TRY(launch_num_dense(b - 7, a, brows + hoff[b], n, o, s));
     }
+    fam("num_dense", "numeric_dense", true, brows + hoff[7], hoff[13] - hoff[7]);
     mk.mark("numeric_dense");
     if (window_bin13) {
       const uint32_t n = hoff[14] - hoff[13];
@@ -321,6 +340,7 @@ struct Run {
         OC_TRY(window_scratch());
         OC_TRY(launch_num_window(a, brows + hoff[13], n, h->max_a_nnz, o, win_scratch, s));
       }
+      fam("num_window", "numeric_window", true, brows + hoff[13], n);
       mk.mark("numeric_window");
     }
     return OC_OK;
@@ -418,6 +438,7 @@ int Run::run_usparse() {
   o.vals = tv;
   o.nnz = nnz;
   OC_TRY(launch_esc(esc_n, acc(), o, s));
+  fam("esc_all_rows", "numeric_usparse_main", true, nullptr, (uint32_t)m);
   mk.mark("numeric_usparse_main");
   if (h->max_products > (uint32_t)esc_n) {
     uint32_t *ovf;
@@ -490,12 +511,15 @@ int Run::run_precise() {
     const uint32_t limit = d.use_est_symbolic ? S - S / 8 : S;
     OC_TRY(launch_sym_hash(b, a, brows + hoff[b], n, limit, nnz, ovf_rows, ovf, s));
   }
+  fam("sym_hash", "symbolic_hash", false, brows + hoff[0], hoff[6] - hoff[0]);
+  mk.mark("symbolic_hash");
   if (hoff[7] > hoff[6]) {
     OC_TRY(window_scratch());
     OC_TRY(launch_sym_window(a, brows + hoff[6], hoff[7] - hoff[6], h->max_a_nnz, nnz,
                              win_scratch, s));
   }
-  mk.mark("symbolic_hash");
+  fam("sym_window", "symbolic_window", false, brows + hoff[6], hoff[7] - hoff[6]);
+  mk.mark("symbolic_window");
   for (int b = 7; b <= 13; ++b) {
     const uint32_t n = hoff[b + 1] - hoff[b];
     if (!n) continue;
@@ -505,6 +529,7 @@ int Run::run_precise() {
     }
     OC_TRY(launch_sym_dense(b - 7, a, brows + hoff[b], n, nnz, s));
   }
+  fam("sym_dense", "symbolic_dense", false, brows + hoff[7], hoff[14] - hoff[7]);
   mk.mark("symbolic_dense");
   if (d.use_est_symbolic) {
     uint32_t n = 0;
@@ -528,7 +553,9 @@ int Run::run_precise() {
   nb.expand_factor = c.numeric_expansion_coe;
   nb.use_largest = d.hash_largest;
   nb.hybrid_max = (uint32_t)(m / 10);
-  OC_TRY(binning(nb, &bor, &brows, &boff, hoff));
+  uint8_t *bor2 = nullptr;
+  uint32_t *brows2 = nullptr, *boff2 = nullptr;
+  OC_TRY(binning(nb, &bor2, &brows2, &boff2, hoff));
   std::memcpy(h->bin2, hoff, sizeof(hoff));
   h->have_bin2 = true;
   mk.mark("numeric_binning");
@@ -539,7 +566,7 @@ int Run::run_precise() {
   o.vals = C->values;
   o.ovf_rows = ovf_rows;
   o.ovf_count = ovf + 1;
-  OC_TRY(numeric_bins(brows, hoff, o, true));
+  OC_TRY(numeric_bins(brows2, hoff, o, true));
   // exact capacities cannot overflow; recount defensively if one did
   uint32_t n = 0;
   OC_TRY(readback(&n, ovf + 1, 1, s));
@@ -639,6 +666,7 @@ int Run::run_est() {
     OC_TRY(window_scratch());
     OC_TRY(launch_num_window(acc(), ovf_rows, novf, h->max_a_nnz, o2, win_scratch, s));
   }
+  fam("num_window", "numeric_overflow", true, ovf_rows, novf);
   mk.mark("numeric_overflow");
   OC_TRY(alloc_c_from_counts(nnz));
   OC_TRY(launch_copy_rows(A.vtype, m, nullptr, nnz, src_off, tc, tv,
@@ -803,6 +831,45 @@ std::string stats_json(const oc_spgemm *h) {
     j.close();
     j.close();
   }
+  if (!h->fams.empty()) {
+    // per launch family: GPU ms and algorithmic bytes (gather model: A
+    // entries + B row pointers, one B entry per product, one C entry per
+    // output, row offsets)
+    const double sv = (double)vsize(h->vtype);
+    j.open("kernels");
+    std::map<std::string, oc_spgemm::FamStat> merged;
+    for (const auto &f : h->fams) {
+      auto it = merged.find(f.name);
+      if (it == merged.end()) merged[f.name] = f;
+      else {
+        it->second.rows += f.rows;
+        it->second.a_nnz += f.a_nnz;
+        it->second.products += f.products;
+        it->second.nnz += f.nnz;
+      }
+    }
+    for (const auto &kv : merged) {
+      const auto &f = kv.second;
+      double bytes;
+      if (f.numeric)
+        bytes = 16.0 * f.rows + (4 + sv + 8) * f.a_nnz + (4 + sv) * (double)f.products +
+                (4 + sv) * (double)f.nnz;
+      else
+        bytes = 12.0 * f.rows + 12.0 * f.a_nnz + 4.0 * (double)f.products;
+      j.open(f.name.c_str());
+      j.num("ms", stage(h, f.stage.c_str()));
+      j.u64("rows", f.rows);
+      j.u64("a_nnz", f.a_nnz);
+      j.u64("products", f.products);
+      j.u64("nnz", f.nnz);
+      j.num("bytes_alg", bytes);
+      j.close();
+    }
+    j.close();
+  }
+  j.open("timing_flat");
+  for (const auto &kv : h->stage_ms) j.num(kv.first.c_str(), kv.second);
+  j.close();
   j.num("total_time", h->total_ms);
   j.num("gflops", (double)(h->total_products * 2) / h->total_ms / 1e6);
   // HBM roofline of the whole product (gather model)
@@ -884,6 +951,27 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
     }
   }
   r.mk.mark("end");
+  h->fams.clear();
+  if (rc == OC_OK && !r.fams.empty() && dC->row_offsets) {
+    unsigned long long *acc = nullptr;
+    const size_t nf = r.fams.size();
+    if (hipMallocAsync((void **)&acc, nf * 32, s) == hipSuccess) {
+      (void)hipMemsetAsync(acc, 0, nf * 32, s);
+      for (size_t i = 0; i < nf && rc == OC_OK; ++i)
+        rc = launch_row_sums(r.fams[i].rows, r.fams[i].n, r.A.rp, r.ri.products,
+                             (const uint64_t *)dC->row_offsets, nullptr, acc + 4 * i, s);
+      std::vector<unsigned long long> hs(4 * nf);
+      if (rc == OC_OK &&
+          hipMemcpyAsync(hs.data(), acc, nf * 32, hipMemcpyDeviceToHost, s) == hipSuccess &&
+          hipStreamSynchronize(s) == hipSuccess) {
+        for (size_t i = 0; i < nf; ++i)
+          h->fams.push_back(oc_spgemm::FamStat{r.fams[i].name, r.fams[i].stage,
+                                               r.fams[i].numeric, hs[4 * i], hs[4 * i + 1],
+                                               hs[4 * i + 2], hs[4 * i + 3]});
+      }
+      (void)hipFreeAsync(acc, s);
+    }
+  }
   hipError_t e = hipStreamSynchronize(s);
   if (rc == OC_OK && e != hipSuccess) {
     set_error("stream error: %s", hipGetErrorString(e));
