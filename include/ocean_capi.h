@@ -118,6 +118,13 @@ typedef struct oc_config {
   /* Additions (no reference counterpart): learned row-size predictor. */
   int32_t use_learned_binning;
   char learned_model_file[512];
+  /* gfx950 kernel choice for rows beyond the LDS hash/dense kernels:
+   * 2 rank-compressed tiles (default), 1 dense tiles, 0 cursor windows. */
+  int32_t use_tiled_numeric;
+  int32_t tile_log2_cols;  /* tile width 2^k columns (10..16), default 14 */
+  int32_t tile_block;      /* workgroup size of the tiled kernel: 256/512/1024 */
+  int32_t tile_products_cap; /* rank tiles: products per tile target (0 = none) */
+  int32_t rank_layout;     /* rank tiles: products per lane 0 strided, 1 contiguous */
 } oc_config;
 
 /* Replaces: Config's default member initialisers (include/Utils.h:22-107). */

@@ -58,6 +58,11 @@ extern "C" void oc_config_default(oc_config *c) {
   c->print_info = 0;
   c->print_memory = 0;
   c->use_learned_binning = 0;
+  c->use_tiled_numeric = 2;
+  c->tile_log2_cols = 14;
+  c->tile_block = 1024;
+  c->tile_products_cap = 0;
+  c->rank_layout = 0;
 }
 
 namespace {
@@ -268,6 +273,11 @@ extern "C" int oc_config_from_json(oc_config *c, const char *text) {
     // learned row-size predictor (no reference counterpart)
     B("use_learned_binning", use_learned_binning)
     S("learned_model_file", learned_model_file)
+    I("use_tiled_numeric", use_tiled_numeric)
+    I("tile_log2_cols", tile_log2_cols)
+    I("tile_block", tile_block)
+    I("tile_products_cap", tile_products_cap)
+    I("rank_layout", rank_layout)
 #undef B
 #undef I
 #undef D

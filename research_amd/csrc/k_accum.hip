@@ -60,6 +60,7 @@ struct KOut {
   const uint32_t *cap;
   uint32_t *ovf_rows;
   uint32_t *ovf_count;
+  const uint32_t *nnz_hint;
 };
 
 template <typename T>
@@ -89,6 +90,7 @@ static KOut<T> kout(const NumOut &o) {
   k.cap = o.cap;
   k.ovf_rows = o.ovf_rows;
   k.ovf_count = o.ovf_count;
+  k.nnz_hint = o.nnz_hint;
   return k;
 }
 
@@ -123,6 +125,45 @@ __device__ __forceinline__ uint32_t lds_load(uint32_t *p) {
 
 __device__ __forceinline__ void lds_add_f64(double *p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Walk one B row segment [s, e) with a group of G lanes (q = s + lig, step
+// G), U iterations at a time: the U column loads are issued before any of
+// them is used, so each lane keeps U gathers in flight.
+template <int U, typename F>
+__device__ __forceinline__ void seg_cols(uint32_t s, uint32_t e, int lig, int G,
+                                         const uint32_t *__restrict__ b_ci, F &&f) {
+  uint32_t q = s + lig;
+  const uint32_t step = (uint32_t)G;
+  for (; q + (U - 1) * step < e; q += U * step) {
+    uint32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = b_ci[q + u * step];
+#pragma unroll
+    for (int u = 0; u < U; ++u) f(c[u]);
+  }
+  for (; q < e; q += step) f(b_ci[q]);
+}
+
+// Same with the B values; f(col, av * b_value).
+template <int U, typename T, typename F>
+__device__ __forceinline__ void seg_vals(uint32_t s, uint32_t e, int lig, int G, double av,
+                                         const uint32_t *__restrict__ b_ci,
+                                         const T *__restrict__ b_v, F &&f) {
+  uint32_t q = s + lig;
+  const uint32_t step = (uint32_t)G;
+  for (; q + (U - 1) * step < e; q += U * step) {
+    uint32_t c[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      c[u] = b_ci[q + u * step];
+      v[u] = b_v[q + u * step];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) f(c[u], av * (double)v[u]);
+  }
+  for (; q < e; q += step) f(b_ci[q], av * (double)b_v[q]);
 }
 
 // =================================================================== ESC
@@ -265,9 +306,8 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
       break;
     }
     const uint32_t k = a.a_ci[j];
-    const uint32_t be = a.b_rp[k + 1];
-    for (uint32_t q = a.b_rp[k] + lig; q < be; q += G) {
-      const uint32_t c = a.b_ci[q];
+    seg_cols<8>(a.b_rp[k], a.b_rp[k + 1], lig, G, a.b_ci, [&](uint32_t c) {
+      if (ovf) return;
       uint32_t slot = hash_slot(c, S);
       uint32_t probes = 0;
       for (; probes < S; ++probes) {
@@ -284,11 +324,8 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
         }
         if (++slot == S) slot = 0;
       }
-      if (probes == S) {
-        ovf = true;
-        break;
-      }
-    }
+      if (probes == S) ovf = true;
+    });
   }
   if (ovf) hdr[1] = 1;
   atomicAdd(&hdr[2], mine);
@@ -324,11 +361,10 @@ __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
   const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
   for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
     const uint32_t k = a.a_ci[j];
-    const uint32_t be = a.b_rp[k + 1];
-    for (uint32_t q = a.b_rp[k] + lig; q < be; q += G) {
-      const uint32_t c = a.b_ci[q] - lo;
+    seg_cols<8>(a.b_rp[k], a.b_rp[k + 1], lig, G, a.b_ci, [&](uint32_t col) {
+      const uint32_t c = col - lo;
       atomicOr(&bits[c >> 5], 1u << (c & 31));
-    }
+    });
   }
   __syncthreads();
   uint32_t cnt = 0;
@@ -405,10 +441,9 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
     }
     const uint32_t k = a.a_ci[j];
     const double av = (double)a.a_v[j];
-    const uint32_t be = a.b_rp[k + 1];
-    for (uint32_t q = a.b_rp[k] + lig; q < be; q += G) {
-      const uint32_t c = a.b_ci[q];
-      const double pv = av * (double)a.b_v[q];
+    seg_vals<4>(a.b_rp[k], a.b_rp[k + 1], lig, G, av, a.b_ci, a.b_v,
+                [&](uint32_t c, double pv) {
+      if (ovf) return;
       uint32_t slot = hash_slot(c, S);
       uint32_t probes = 0;
       for (; probes < S; ++probes) {
@@ -426,10 +461,10 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
       }
       if (probes == S) {
         ovf = true;
-        break;
+        return;
       }
       lds_add_f64(&vals[slot], pv);
-    }
+    });
   }
   if (ovf) hdr[1] = 1;
   __syncthreads();
@@ -495,12 +530,12 @@ __global__ __launch_bounds__(BS) void k_num_dense(KArgs<T> a,
   for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
     const uint32_t k = a.a_ci[j];
     const double av = (double)a.a_v[j];
-    const uint32_t be = a.b_rp[k + 1];
-    for (uint32_t q = a.b_rp[k] + lig; q < be; q += G) {
-      const uint32_t c = a.b_ci[q] - lo;
-      lds_add_f64(&vals[c], av * (double)a.b_v[q]);
+    seg_vals<4>(a.b_rp[k], a.b_rp[k + 1], lig, G, av, a.b_ci, a.b_v,
+                [&](uint32_t col, double pv) {
+      const uint32_t c = col - lo;
+      lds_add_f64(&vals[c], pv);
       atomicOr(&bits[c >> 5], 1u << (c & 31));
-    }
+    });
   }
   __syncthreads();
   const uint32_t wpt = (nwords + BS - 1) / BS;
@@ -640,6 +675,545 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
   }
 }
 
+template <typename K>
+static int set_lds(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    HIP_TRY(hipFuncSetAttribute((const void *)kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+  return OC_OK;
+}
+
+// ================================================================= tiled
+// Column tiles of W = 2^logW columns.  ts[k * T1 + t] is the index of the
+// first entry of B row k whose column is >= t * W (T1 = tiles + 1), so the
+// part of any B row inside any tile is known without walking the row.
+__global__ __launch_bounds__(256) void k_tile_starts(const uint32_t *__restrict__ b_rp,
+                                                     const uint32_t *__restrict__ b_ci,
+                                                     uint32_t rows, uint32_t logW,
+                                                     uint32_t T1, uint32_t *__restrict__ ts) {
+  const int lane = lane_id();
+  for (uint32_t k = blockIdx.x * 4 + threadIdx.x / kWave; k < rows; k += gridDim.x * 4) {
+    const uint32_t s = b_rp[k], e = b_rp[k + 1];
+    for (uint32_t t = lane; t < T1; t += kWave) {
+      const uint64_t bound = (uint64_t)t << logW;
+      uint32_t lo = s, hi = e;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)b_ci[mid] < bound) lo = mid + 1;
+        else hi = mid;
+      }
+      ts[(uint64_t)k * T1 + t] = lo;
+    }
+  }
+}
+
+// Numeric rows swept tile by tile (one 1024-thread workgroup per row,
+// rows taken from an atomic counter).  For each tile the segments of the
+// referenced B rows come from the tile-start table; their products are
+// spread evenly over all lanes (prefix sum of segment lengths + binary
+// search for the owning A entry), so every lane issues independent
+// gathers.  FP64 sums + bitmap over the tile in LDS; the tile is written
+// out in column order (the role of kernels/AccumulatorDense.cuh:610-806).
+static size_t tile_lds(uint32_t W, uint32_t bs) {
+  return 256 + (size_t)bs * (4 + 4 + 4 + 8) + (W / 32) * 4 + (size_t)W * 8;
+}
+
+template <typename T, int kTileBlock>
+__global__ __launch_bounds__(kTileBlock) void k_num_tiles(
+    KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
+    uint32_t logW, const uint32_t *__restrict__ ts, uint32_t T1, KOut<T> o,
+    const uint32_t *nrows_dev) {
+  constexpr uint32_t kTileA = kTileBlock;  // A entries staged per pass
+  if (nrows_dev) nrows = *nrows_dev;
+  extern __shared__ __align__(16) uint8_t smem[];
+  const uint32_t W = 1u << logW;
+  uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
+  uint32_t *sk = reinterpret_cast<uint32_t *>(smem + 256);
+  uint32_t *sst = sk + kTileA;
+  uint32_t *spre = sst + kTileA;
+  double *sav = reinterpret_cast<double *>(spre + kTileA);
+  uint32_t *bits = reinterpret_cast<uint32_t *>(sav + kTileA);
+  double *vals = reinterpret_cast<double *>(bits + W / 32);
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < W / 32; i += kTileBlock) bits[i] = 0;
+  for (uint32_t i = tid; i < W; i += kTileBlock) vals[i] = 0.0;
+  __syncthreads();
+  for (;;) {
+    if (tid == 0) hdr[0] = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t idx = hdr[0];
+    __syncthreads();
+    if (idx >= nrows) break;
+    const uint32_t r = rows[idx];
+    const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1], nA = a1 - a0;
+    const uint32_t left = a.left[r], hi = a.right[r];
+    if (nA == 0 || a.products[r] == 0 || hi < left) {
+      if (tid == 0 && o.nnz) o.nnz[r] = 0;
+      continue;
+    }
+    const bool staged = nA <= kTileA;
+    if (staged) {
+      for (uint32_t j = tid; j < nA; j += kTileBlock) {
+        sk[j] = a.a_ci[a0 + j];
+        sav[j] = (double)a.a_v[a0 + j];
+      }
+      __syncthreads();
+    }
+    const uint64_t off = o.off[r];
+    uint64_t total = 0;
+    const uint32_t t_begin = left >> logW, t_end = hi >> logW;
+    for (uint32_t t = t_begin; t <= t_end; ++t) {
+      const uint64_t wbase = (uint64_t)t << logW;
+      uint32_t any = 0;
+      for (uint32_t c0 = a0; c0 < a1; c0 += kTileA) {
+        const uint32_t nc = min(kTileA, a1 - c0);
+        if (!staged) {
+          if (tid < nc) {
+            sk[tid] = a.a_ci[c0 + tid];
+            sav[tid] = (double)a.a_v[c0 + tid];
+          }
+          __syncthreads();
+        }
+        uint32_t len = 0, st = 0;
+        if (tid < nc) {
+          const uint64_t base = (uint64_t)sk[tid] * T1 + t;
+          st = ts[base];
+          len = ts[base + 1] - st;
+        }
+        uint32_t P;
+        const uint32_t ex = block_excl_sum(len, hdr + 8, &P);
+        if (tid < nc) {
+          sst[tid] = st;
+          spre[tid] = ex + len;
+        }
+        __syncthreads();
+        any |= P;
+        // flattened gathers, 4 products per lane in flight
+        for (uint32_t p0 = 0; p0 < P; p0 += 4 * kTileBlock) {
+          uint32_t col[4];
+          T bv[4];
+          double av[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t p = p0 + u * kTileBlock + tid;
+            col[u] = kEmpty;
+            if (p < P) {
+              uint32_t lo = 0, hb = nc - 1;
+              while (lo < hb) {
+                const uint32_t mid = (lo + hb) >> 1;
+                if (spre[mid] > p) hb = mid;
+                else lo = mid + 1;
+              }
+              const uint32_t prev = lo ? spre[lo - 1] : 0u;
+              const uint32_t q = sst[lo] + (p - prev);
+              col[u] = a.b_ci[q];
+              bv[u] = a.b_v[q];
+              av[u] = sav[lo];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (col[u] != kEmpty) {
+              const uint32_t x = (uint32_t)(col[u] - wbase);
+              lds_add_f64(&vals[x], av[u] * (double)bv[u]);
+              atomicOr(&bits[x >> 5], 1u << (x & 31));
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (!any) continue;
+      // ordered write-out of the tile (W/32 words, one or more per thread)
+      const uint32_t nwords = W / 32;
+      const uint32_t wpt = (nwords + kTileBlock - 1) / kTileBlock;
+      const uint32_t b0 = min(nwords, tid * wpt), b1 = min(nwords, b0 + wpt);
+      uint32_t mycnt = 0;
+      for (uint32_t i = b0; i < b1; ++i) mycnt += __popc(bits[i]);
+      uint32_t wtotal;
+      const uint32_t base = block_excl_sum(mycnt, hdr + 8, &wtotal);
+      uint64_t pos = off + total + base;
+      for (uint32_t i = b0; i < b1; ++i) {
+        uint32_t m = bits[i];
+        bits[i] = 0;
+        while (m) {
+          const int b = __ffs(m) - 1;
+          m &= m - 1;
+          const uint32_t x = i * 32 + b;
+          o.cols[pos] = (uint32_t)(wbase + x);
+          o.vals[pos] = (T)vals[x];
+          vals[x] = 0.0;
+          ++pos;
+        }
+      }
+      total += wtotal;
+      __syncthreads();
+    }
+    if (tid == 0 && o.nnz) o.nnz[r] = (uint32_t)total;
+  }
+}
+
+// Rank-compressed tiles for rows beyond the LDS hash/dense kernels.  A
+// tile of W = 2^lw columns (lw from the row's density, 2^14..2^17) is done
+// in two passes over its products: the first marks the output columns in
+// an LDS bitmap, a popcount prefix over the bitmap words then gives every
+// output column its rank, and the second pass adds each product into an
+// FP64 slot indexed by that rank.  Sums need one slot per output entry, not
+// per column, so tiles are wide and few; ranks are the output order, so the
+// tile is written with coalesced stores.  Tiles with more outputs than
+// slots are finished in rank ranges.  Segment bounds come from the
+// tile-start table (granularity 2^logT columns).
+constexpr int kRankBlock = 1024;
+constexpr uint32_t kRankA = 1024;
+constexpr uint32_t kRankMaxLog = 17;
+constexpr uint32_t kRankWords = (1u << kRankMaxLog) / 32;
+
+static size_t rank_lds_fixed() {
+  return 256 + (size_t)kRankA * (4 + 4 + 4 + 8) + (size_t)kRankWords * 8;
+}
+
+__device__ __forceinline__ uint32_t owner_of(const uint32_t *spre, uint32_t nc, uint32_t p) {
+  uint32_t lo = 0, hb = nc - 1;
+  while (lo < hb) {
+    const uint32_t mid = (lo + hb) >> 1;
+    if (spre[mid] > p) hb = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Products pb .. pb+7 (< P) of one lane in a chunk's flattened product
+// space: B entry q[u] and owning A entry w[u].  One binary search for the
+// first, then the owner only advances at segment ends.
+__device__ __forceinline__ void locate8(const uint32_t *spre, const uint32_t *sst,
+                                        uint32_t nc, uint32_t pb, uint32_t P,
+                                        uint32_t (&q)[8], uint32_t (&w)[8]) {
+  uint32_t o = pb < P ? owner_of(spre, nc, pb) : 0u;
+  uint32_t e = spre[o];
+  uint32_t sb = o ? spre[o - 1] : 0u;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const uint32_t p = pb + u;
+    q[u] = kEmpty;
+    w[u] = 0;
+    if (p < P) {
+      while (p >= e) {
+        ++o;
+        sb = e;
+        e = spre[o];
+      }
+      q[u] = sst[o] + (p - sb);
+      w[u] = o;
+    }
+  }
+}
+
+// Lane-coalesced variant: products pb + u*1024 (one binary search each).
+__device__ __forceinline__ void locate8_strided(const uint32_t *spre, const uint32_t *sst,
+                                                uint32_t nc, uint32_t pb, uint32_t P,
+                                                uint32_t (&q)[8], uint32_t (&w)[8]) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const uint32_t p = pb + u * 1024u;
+    q[u] = kEmpty;
+    w[u] = 0;
+    if (p < P) {
+      const uint32_t o = owner_of(spre, nc, p);
+      q[u] = sst[o] + (p - (o ? spre[o - 1] : 0u));
+      w[u] = o;
+    }
+  }
+}
+
+// OR the column bits of up to 8 products into the tile bitmap, combining
+// products that fall into the same 32-column word.
+__device__ __forceinline__ void mark8(const uint32_t (&col)[8], uint64_t wbase, uint32_t *bits) {
+  uint32_t cw = kEmpty, cm = 0;
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if (col[u] != kEmpty) {
+      const uint32_t x = (uint32_t)(col[u] - wbase);
+      if ((x >> 5) != cw) {
+        if (cm) atomicOr(&bits[cw], cm);
+        cw = x >> 5;
+        cm = 0;
+      }
+      cm |= 1u << (x & 31);
+    }
+  if (cm) atomicOr(&bits[cw], cm);
+}
+
+__device__ __forceinline__ uint32_t rank_of(uint32_t x, const uint32_t *wpre, const uint32_t *bits) {
+  const uint32_t wd = x >> 5;
+  return wpre[wd] + __popc(bits[wd] & ((1u << (x & 31)) - 1u));
+}
+
+template <typename T, bool CONTIG>
+__global__ __launch_bounds__(kRankBlock) void k_num_rank(
+    KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
+    uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1, uint32_t cap, uint32_t pcap,
+    KOut<T> o) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
+  uint32_t *sk = reinterpret_cast<uint32_t *>(smem + 256);
+  uint32_t *sst = sk + kRankA;
+  uint32_t *spre = sst + kRankA;
+  double *sav = reinterpret_cast<double *>(spre + kRankA);
+  uint32_t *bits = reinterpret_cast<uint32_t *>(sav + kRankA);
+  uint32_t *wpre = bits + kRankWords;
+  double *acc = reinterpret_cast<double *>(wpre + kRankWords);
+  uint32_t *cmap = reinterpret_cast<uint32_t *>(acc + cap);
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < kRankWords; i += kRankBlock) bits[i] = 0;
+  for (uint32_t i = tid; i < cap; i += kRankBlock) acc[i] = 0.0;
+  __syncthreads();
+  for (;;) {
+    if (tid == 0) hdr[0] = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t idx = hdr[0];
+    __syncthreads();
+    if (idx >= nrows) break;
+    const uint32_t r = rows[idx];
+    const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1], nA = a1 - a0;
+    const uint32_t left = a.left[r], hi = a.right[r];
+    if (nA == 0 || a.products[r] == 0 || hi < left) {
+      if (tid == 0 && o.nnz) o.nnz[r] = 0;
+      continue;
+    }
+    // tile width from the row density: about 3/4 of the slots per tile
+    const uint64_t span = (uint64_t)hi - left + 1;
+    uint64_t est = a.products[r] < span ? a.products[r] : span;
+    if (o.nnz_hint) est = o.nnz_hint[r];
+    const uint64_t prod = a.products[r];
+    uint32_t lw = logT;
+    while (lw < kRankMaxLog && (uint64_t)(span >> lw) >= 1 &&
+           est * (1ull << (lw + 1)) <= (uint64_t)cap * 3 / 4 * span &&
+           (pcap == 0 || prod * (1ull << (lw + 1)) <= (uint64_t)pcap * span))
+      ++lw;
+    const uint32_t step = lw - logT;
+    const bool one_chunk = nA <= kRankA;
+    const uint64_t off = o.off[r];
+    uint64_t total = 0;
+    const uint32_t t_begin = left >> lw, t_end = hi >> lw;
+    const uint32_t nwords = 1u << (lw - 5);
+    const uint32_t wpt = (nwords + kRankBlock - 1) / kRankBlock;
+    const uint32_t b0 = min(nwords, tid * wpt), b1 = min(nwords, b0 + wpt);
+
+    // stage chunk c0.. of the A row and the segment bounds of tile t
+    auto stage = [&](uint32_t c0, uint32_t nc) {
+      if (tid < nc) {
+        sk[tid] = a.a_ci[c0 + tid];
+        sav[tid] = (double)a.a_v[c0 + tid];
+      }
+      __syncthreads();
+    };
+    auto bounds = [&](uint32_t t, uint32_t nc, uint32_t &P) {
+      uint32_t len = 0, st = 0;
+      if (tid < nc) {
+        const uint64_t base = (uint64_t)sk[tid] * T1;
+        st = ts[base + (t << step)];
+        len = ts[base + min(T1 - 1, (t + 1) << step)] - st;
+      }
+      const uint32_t ex = block_excl_sum(len, hdr + 8, &P);
+      if (tid < nc) {
+        sst[tid] = st;
+        spre[tid] = ex + len;
+      }
+      __syncthreads();
+    };
+    if (one_chunk) stage(a0, nA);
+    for (uint32_t t = t_begin; t <= t_end; ++t) {
+      const uint64_t wbase = (uint64_t)t << lw;
+      // ---- pass 1: mark the output columns
+      uint32_t any = 0;
+      bool keep = false;
+      uint32_t kc[8];
+      double kv[8];
+      for (uint32_t c0 = a0; c0 < a1; c0 += kRankA) {
+        const uint32_t nc = min(kRankA, a1 - c0);
+        if (!one_chunk) stage(c0, nc);
+        uint32_t P;
+        bounds(t, nc, P);
+        any |= P;
+        keep = one_chunk && P <= 8 * kRankBlock;
+        for (uint32_t p0 = 0; p0 < P; p0 += 8 * kRankBlock) {
+          uint32_t q[8], w[8];
+          if (CONTIG) locate8(spre, sst, nc, p0 + 8 * tid, P, q, w);
+          else locate8_strided(spre, sst, nc, p0 + tid, P, q, w);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            kc[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
+            if (keep) kv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
+          }
+          mark8(kc, wbase, bits);
+        }
+        if (!one_chunk) __syncthreads();
+      }
+      if (!any) continue;
+      __syncthreads();
+      // ---- ranks: exclusive popcount prefix per bitmap word
+      uint32_t mycnt = 0;
+      for (uint32_t i = b0; i < b1; ++i) mycnt += __popc(bits[i]);
+      uint32_t m;
+      uint32_t run = block_excl_sum(mycnt, hdr + 8, &m);
+      for (uint32_t i = b0; i < b1; ++i) {
+        wpre[i] = run;
+        run += __popc(bits[i]);
+      }
+      __syncthreads();
+      // ---- pass 2: sums by rank (in rank ranges when m > cap)
+      for (uint32_t r0 = 0; r0 < m; r0 += cap) {
+        const uint32_t r1 = min(m, r0 + cap);
+        if (keep) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (kc[u] != kEmpty) {
+              const uint32_t rk = rank_of((uint32_t)(kc[u] - wbase), wpre, bits);
+              if (rk >= r0 && rk < r1) {
+                lds_add_f64(&acc[rk - r0], kv[u]);
+                cmap[rk - r0] = kc[u];
+              }
+            }
+        } else {
+          for (uint32_t c0 = a0; c0 < a1; c0 += kRankA) {
+            const uint32_t nc = min(kRankA, a1 - c0);
+            uint32_t P = spre[nc - 1];
+            if (!one_chunk) {
+              stage(c0, nc);
+              bounds(t, nc, P);
+            }
+            for (uint32_t p0 = 0; p0 < P; p0 += 8 * kRankBlock) {
+              uint32_t q[8], w[8];
+              if (CONTIG) locate8(spre, sst, nc, p0 + 8 * tid, P, q, w);
+              else locate8_strided(spre, sst, nc, p0 + tid, P, q, w);
+              uint32_t col[8];
+              double pv[8];
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                col[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
+                pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
+              }
+#pragma unroll
+              for (int u = 0; u < 8; ++u)
+                if (col[u] != kEmpty) {
+                  const uint32_t rk = rank_of((uint32_t)(col[u] - wbase), wpre, bits);
+                  if (rk >= r0 && rk < r1) {
+                    lds_add_f64(&acc[rk - r0], pv[u]);
+                    cmap[rk - r0] = col[u];
+                  }
+                }
+            }
+            if (!one_chunk) __syncthreads();
+          }
+        }
+        __syncthreads();
+        // coalesced write-out of ranks [r0, r1)
+        for (uint32_t i = tid; i < r1 - r0; i += kRankBlock) {
+          o.cols[off + total + r0 + i] = cmap[i];
+          o.vals[off + total + r0 + i] = (T)acc[i];
+          acc[i] = 0.0;
+        }
+        __syncthreads();
+      }
+      for (uint32_t i = b0; i < b1; ++i) bits[i] = 0;
+      total += m;
+      __syncthreads();
+    }
+    if (tid == 0 && o.nnz) o.nnz[r] = (uint32_t)total;
+  }
+}
+
+int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
+                    const uint32_t *ts, uint32_t T1, uint32_t pcap, int contig,
+                    const NumOut &o, void *scratch, hipStream_t s) {
+  if (nrows == 0) return OC_OK;
+  const Tables &t = tables();
+  const size_t fixed = rank_lds_fixed();
+  if (fixed + 64 * 12 > t.lds_max_block) {
+    set_error("rank-tile kernel does not fit the LDS of this device");
+    return OC_ERR_INVALID;
+  }
+  const uint32_t cap = (uint32_t)(((t.lds_max_block - fixed) / 12) & ~63ull);
+  const size_t lds = fixed + (size_t)cap * 12;
+  uint32_t *counter = (uint32_t *)scratch;
+  HIP_TRY(hipMemsetAsync(counter, 0, 4, s));
+  const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu));
+#define GO(T, C)                                                                    \
+  do {                                                                              \
+    OC_TRY(set_lds(k_num_rank<T, C>, lds));                                         \
+    hipLaunchKernelGGL((k_num_rank<T, C>), dim3(grid), dim3(kRankBlock), lds, s,    \
+                       kargs<T>(a), rows, nrows, counter, logT, ts, T1, cap, pcap,  \
+                       kout<T>(o));                                                 \
+  } while (0)
+  if (a.A.vtype == OC_F64) {
+    if (contig) GO(double, true);
+    else GO(double, false);
+  } else {
+    if (contig) GO(float, true);
+    else GO(float, false);
+  }
+#undef GO
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
+int tile_plan(uint64_t b_rows, uint64_t b_cols, uint32_t log2_cols, uint32_t bs,
+              uint32_t *logW, uint32_t *T1, size_t *ts_bytes) {
+  const Tables &t = tables();
+  uint32_t lw = log2_cols;
+  if (lw < 10 || lw > 16 || (bs != 256 && bs != 512 && bs != 1024)) return 0;
+  if (tile_lds(1u << lw, bs) > t.lds_max_block) return 0;
+  const uint64_t tiles = b_cols ? ((b_cols - 1) >> lw) + 1 : 1;
+  const uint64_t bytes = b_rows * (tiles + 1) * 4;
+  if (tiles > 4096 || bytes > (8ull << 30)) return 0;
+  *logW = lw;
+  *T1 = (uint32_t)(tiles + 1);
+  *ts_bytes = (size_t)bytes;
+  return 1;
+}
+
+int launch_tile_starts(const DevCsr &B, uint32_t logW, uint32_t T1, uint32_t *ts,
+                       hipStream_t s) {
+  if (B.rows == 0) return OC_OK;
+  hipLaunchKernelGGL(k_tile_starts, dim3(std::min<uint32_t>(cdiv(B.rows, 4), 1u << 20)),
+                     dim3(256), 0, s, B.rp, B.ci, (uint32_t)B.rows, logW, T1, ts);
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
+int launch_num_tiles(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logW,
+                     uint32_t bs, const uint32_t *ts, uint32_t T1, const NumOut &o,
+                     void *scratch, hipStream_t s, const uint32_t *nrows_dev) {
+  if (nrows == 0) return OC_OK;
+  const Tables &t = tables();
+  const size_t lds = tile_lds(1u << logW, bs);
+  uint32_t *counter = (uint32_t *)scratch;
+  HIP_TRY(hipMemsetAsync(counter, 0, 4, s));
+  // persistent: as many workgroups as fit on the chip at once
+  const uint32_t per_cu = std::max<uint32_t>(
+      1, std::min<uint32_t>(2048 / bs, (uint32_t)(t.lds_per_cu / lds)));
+  const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu) * per_cu);
+#define GO(T, BS)                                                                   \
+  do {                                                                              \
+    OC_TRY(set_lds(k_num_tiles<T, BS>, lds));                                       \
+    hipLaunchKernelGGL((k_num_tiles<T, BS>), dim3(grid), dim3(BS), lds, s,          \
+                       kargs<T>(a), rows, nrows, counter, logW, ts, T1, kout<T>(o),  \
+                       nrows_dev);                                                   \
+  } while (0)
+  if (a.A.vtype == OC_F64) {
+    if (bs == 256) GO(double, 256);
+    else if (bs == 512) GO(double, 512);
+    else GO(double, 1024);
+  } else {
+    if (bs == 256) GO(float, 256);
+    else if (bs == 512) GO(float, 512);
+    else GO(float, 1024);
+  }
+#undef GO
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
 size_t window_scratch_bytes(uint32_t max_a_nnz) {
   const Tables &t = tables();
   uint64_t grid = std::max<uint32_t>(1, t.num_cu);
@@ -651,14 +1225,6 @@ static size_t window_lds(uint32_t W, bool sym) {
   return 256 + kWinLdsCursors * 4 + (W / 32) * 4 + (sym ? 0 : (size_t)W * 8);
 }
 
-template <typename K>
-static int set_lds(K kernel, size_t bytes) {
-  if (bytes > 65536)
-    HIP_TRY(hipFuncSetAttribute((const void *)kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)bytes));
-  return OC_OK;
-}
 
 template <typename T, bool SYM>
 static int window_go(const AccArgs &a, const uint32_t *rows, uint32_t nrows,

@@ -116,6 +116,7 @@ struct NumOut {
   const uint32_t *cap;     // per-row capacity (est mode) or null
   uint32_t *ovf_rows;      // est mode: overflow row list
   uint32_t *ovf_count;
+  const uint32_t *nnz_hint;  // exact per-row counts when known (or null)
 };
 
 // Symbolic: rows of one bin.  Overflowing rows (estimated-compaction hash
@@ -139,6 +140,21 @@ int launch_num_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
 int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                       uint32_t max_a_nnz, const NumOut &o, void *scratch,
                       hipStream_t s);
+// Tiled numeric path for rows too large for LDS hash / dense kernels:
+// tile_plan picks the tile width (2^logW columns) and the size of the
+// tile-start table (B.rows x T1 u32); returns 0 when tiles do not apply.
+int tile_plan(uint64_t b_rows, uint64_t b_cols, uint32_t log2_cols, uint32_t bs,
+              uint32_t *logW, uint32_t *T1, size_t *ts_bytes);
+int launch_tile_starts(const DevCsr &B, uint32_t logW, uint32_t T1, uint32_t *ts,
+                       hipStream_t s);
+int launch_num_tiles(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logW,
+                     uint32_t bs, const uint32_t *ts, uint32_t T1, const NumOut &o,
+                     void *scratch, hipStream_t s, const uint32_t *nrows_dev = nullptr);
+// Rank-compressed tiles (tile-start table of granularity 2^logT columns).
+// contig: products laid out contiguously per lane (1) or lane-coalesced (0).
+int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
+                    const uint32_t *ts, uint32_t T1, uint32_t pcap, int contig,
+                    const NumOut &o, void *scratch, hipStream_t s);
 // Ultra-sparse expand-sort-compress for all rows with products <= esc_n.
 int launch_esc(int esc_n, const AccArgs &a, const NumOut &o, hipStream_t s);
 

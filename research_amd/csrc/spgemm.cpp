@@ -241,6 +241,9 @@ struct Run {
   void *scan_tmp = nullptr;
   uint8_t *hll_b = nullptr;
   void *win_scratch = nullptr;
+  uint32_t *tile_ts = nullptr;
+  uint32_t tile_logw = 0, tile_t1 = 0;
+  int tiles_ok = -1;
   oc_csr *C;
   struct Fam {
     const char *name, *stage;
@@ -274,6 +277,35 @@ struct Run {
       win_scratch = p;
     }
     return OC_OK;
+  }
+
+  // Tile-start table of B for the tiled numeric kernel (built once).
+  int tiles() {
+    if (tiles_ok >= 0) return OC_OK;
+    size_t bytes = 0;
+    tiles_ok = (h->cfg.use_tiled_numeric != 0) &&
+               tile_plan(B.rows, B.cols, (uint32_t)h->cfg.tile_log2_cols,
+                         (uint32_t)h->cfg.tile_block, &tile_logw, &tile_t1, &bytes);
+    if (tiles_ok) {
+      OC_TRY(ar.get(&tile_ts, bytes / 4));
+      OC_TRY(launch_tile_starts(B, tile_logw, tile_t1, tile_ts, s));
+      mk.mark("numeric_tile_starts");
+    }
+    return OC_OK;
+  }
+
+  int big_rows(const uint32_t *rows, uint32_t n, const NumOut &o) {
+    if (!n) return OC_OK;
+    OC_TRY(window_scratch());
+    OC_TRY(tiles());
+    if (tiles_ok && h->cfg.use_tiled_numeric == 2)
+      return launch_num_rank(acc(), rows, n, tile_logw, tile_ts, tile_t1,
+                             (uint32_t)h->cfg.tile_products_cap, h->cfg.rank_layout, o,
+                             win_scratch, s);
+    if (tiles_ok)
+      return launch_num_tiles(acc(), rows, n, tile_logw, (uint32_t)h->cfg.tile_block, tile_ts,
+                              tile_t1, o, win_scratch, s);
+    return launch_num_window(acc(), rows, n, h->max_a_nnz, o, win_scratch, s);
   }
 
   // C: u64 offsets (rows+1) from per-row counts, then cols/values.
@@ -336,10 +368,9 @@ struct Run {
     mk.mark("numeric_dense");
     if (window_bin13) {
       const uint32_t n = hoff[14] - hoff[13];
-      if (n) {
-        OC_TRY(window_scratch());
-        OC_TRY(launch_num_window(a, brows + hoff[13], n, h->max_a_nnz, o, win_scratch, s));
-      }
+      if (n) OC_TRY(tiles());
+      mk.mark("numeric_tile_starts");
+      OC_TRY(big_rows(brows + hoff[13], n, o));
       fam("num_window", "numeric_window", true, brows + hoff[13], n);
       mk.mark("numeric_window");
     }
@@ -566,15 +597,13 @@ int Run::run_precise() {
   o.vals = C->values;
   o.ovf_rows = ovf_rows;
   o.ovf_count = ovf + 1;
+  o.nnz_hint = nnz;
   OC_TRY(numeric_bins(brows2, hoff, o, true));
   // exact capacities cannot overflow; recount defensively if one did
   uint32_t n = 0;
   OC_TRY(readback(&n, ovf + 1, 1, s));
   h->precise_recount = n;
-  if (n) {
-    OC_TRY(window_scratch());
-    OC_TRY(launch_num_window(a, ovf_rows, n, h->max_a_nnz, o, win_scratch, s));
-  }
+  OC_TRY(big_rows(ovf_rows, n, o));
   return OC_OK;
 }
 
@@ -663,8 +692,7 @@ int Run::run_est() {
     NumOut o2 = o;
     o2.off = src_off;
     o2.cap = nullptr;
-    OC_TRY(window_scratch());
-    OC_TRY(launch_num_window(acc(), ovf_rows, novf, h->max_a_nnz, o2, win_scratch, s));
+    OC_TRY(big_rows(ovf_rows, novf, o2));
   }
   fam("num_window", "numeric_overflow", true, ovf_rows, novf);
   mk.mark("numeric_overflow");

@@ -66,6 +66,9 @@ class _Config(C.Structure):
         ("warmup_iters", C.c_int32), ("bench_iters", C.c_int32),
         ("print_info", C.c_int32), ("print_memory", C.c_int32),
         ("use_learned_binning", C.c_int32), ("learned_model_file", C.c_char * 512),
+        ("use_tiled_numeric", C.c_int32), ("tile_log2_cols", C.c_int32),
+        ("tile_block", C.c_int32), ("tile_products_cap", C.c_int32),
+        ("rank_layout", C.c_int32),
     ]
 
 

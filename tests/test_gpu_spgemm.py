@@ -77,3 +77,16 @@ def test_precise_dense_and_hash_bins(gpu):
     C, st = _run(gpu, A, A, gpu.Config(use_ultrasparse_workflow=0), 0)
     check_product(A, A, C, TOL64, "er-precise")
     log(f"bins: {st.get('binning_1')} / {st.get('binning_2')}")
+
+
+@pytest.mark.parametrize("tiled", [2, 1, 0])
+def test_big_rows_tiled_and_windowed(gpu, tiled):
+    """Rows beyond the LDS hash/dense kernels (numeric bin 13) through the
+    tiled kernel and through the cursor-windowed kernel."""
+    log(f"case: rmat 2^15 deg 32 f64, big rows, use_tiled_numeric={tiled}")
+    A = gpu.CSR.rmat(15, 32 << 15, seed=5)
+    cfg = gpu.Config(use_tiled_numeric=tiled, track_stage_times=1)
+    C, st = _run(gpu, A, A, cfg)
+    assert st["binning_2"][14] - st["binning_2"][13] > 0, "no rows reached the big-row bin"
+    worst, plain = check_product(A, A, C, TOL64, f"bigrows tiled={tiled}")
+    log(f"tiled={tiled}: big rows {st['binning_2'][14] - st['binning_2'][13]} err {worst:.2e}")
