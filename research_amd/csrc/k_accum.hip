@@ -993,6 +993,10 @@ __global__ __launch_bounds__(kRankBlock) void k_num_rank(
     const uint32_t step = lw - logT;
     const bool one_chunk = nA <= kRankA;
     const uint64_t off = o.off[r];
+    // estimated capacity (estimation workflow): a row whose outputs exceed
+    // it stops before writing past its region and joins the overflow list
+    const uint64_t rcap = o.cap ? (uint64_t)o.cap[r] : ~0ull;
+    bool overflow = false;
     uint64_t total = 0;
     const uint32_t t_begin = left >> lw, t_end = hi >> lw;
     const uint32_t nwords = 1u << (lw - 5);
@@ -1061,6 +1065,12 @@ __global__ __launch_bounds__(kRankBlock) void k_num_rank(
         run += __popc(bits[i]);
       }
       __syncthreads();
+      if (total + m > rcap) {  // m and total are uniform over the block
+        for (uint32_t i = b0; i < b1; ++i) bits[i] = 0;
+        __syncthreads();
+        overflow = true;
+        break;
+      }
       // ---- pass 2: sums by rank (in rank ranges when m > cap)
       for (uint32_t r0 = 0; r0 < m; r0 += cap) {
         const uint32_t r1 = min(m, r0 + cap);
@@ -1119,7 +1129,10 @@ __global__ __launch_bounds__(kRankBlock) void k_num_rank(
       total += m;
       __syncthreads();
     }
-    if (tid == 0 && o.nnz) o.nnz[r] = (uint32_t)total;
+    if (tid == 0) {
+      if (overflow) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
+      if (o.nnz) o.nnz[r] = overflow ? 0u : (uint32_t)total;
+    }
   }
 }
 

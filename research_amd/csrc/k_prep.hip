@@ -820,25 +820,29 @@ int launch_copy_rows(int vtype, uint64_t nrows, const uint32_t *rows,
 // 612-617); its output offset becomes base + reservation.
 __global__ void k_assign_spare(const uint32_t *__restrict__ ovf_rows, uint32_t n,
                                const uint32_t *__restrict__ products,
-                               uint64_t cols, uint64_t base,
+                               const uint32_t *__restrict__ left,
+                               const uint32_t *__restrict__ right, uint64_t base,
                                unsigned long long *spare_used,
                                uint64_t *__restrict__ off) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
     const uint32_t r = ovf_rows[i];
+    // a row has at most min(products, column span) outputs
+    const uint64_t span = right[r] >= left[r] ? (uint64_t)right[r] - left[r] + 1 : 0;
     const unsigned long long need =
-        (unsigned long long)min((uint64_t)products[r], cols);
+        (unsigned long long)min((uint64_t)products[r], span);
     off[r] = base + atomicAdd(spare_used, need);
   }
 }
 
 int launch_assign_spare(const uint32_t *ovf_rows, uint32_t n,
-                        const uint32_t *products, uint64_t cols, uint64_t base,
+                        const uint32_t *products, const uint32_t *left,
+                        const uint32_t *right, uint64_t base,
                         unsigned long long *spare_used, uint64_t *off,
                         hipStream_t s) {
   if (n == 0) return OC_OK;
   hipLaunchKernelGGL(k_assign_spare, dim3(std::min<uint32_t>(cdiv(n, 256), 4096)),
-                     dim3(256), 0, s, ovf_rows, n, products, cols, base,
+                     dim3(256), 0, s, ovf_rows, n, products, left, right, base,
                      spare_used, off);
   LAUNCH_CHECK();
   return OC_OK;

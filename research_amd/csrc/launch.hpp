@@ -86,7 +86,8 @@ int launch_copy_rows(int vtype, uint64_t nrows, const uint32_t *rows,
 
 // Reserve spare space for overflow rows; off[row] = base + reservation.
 int launch_assign_spare(const uint32_t *ovf_rows, uint32_t n,
-                        const uint32_t *products, uint64_t cols, uint64_t base,
+                        const uint32_t *products, const uint32_t *left,
+                        const uint32_t *right, uint64_t base,
                         unsigned long long *spare_used, uint64_t *off,
                         hipStream_t s);
 

@@ -663,15 +663,26 @@ int Run::run_est() {
   o.cap = est;
   o.ovf_rows = ovf_rows;
   o.ovf_count = ovf;
+  o.nnz_hint = est;
   OC_TRY(numeric_bins(brows, hoff, o, false));
-  // overflow rows plus the windowed-bin rows go to the spare region
+  // Long rows (bin 13): the rank-tile kernel checks the estimate and works
+  // in place; the other long-row kernels have no capacity check, so their
+  // rows go to the spare region with the overflow rows.
+  const uint32_t n13 = hoff[14] - hoff[13];
+  if (n13) OC_TRY(tiles());
+  const bool inplace13 = n13 && tiles_ok && c.use_tiled_numeric == 2;
+  if (inplace13) {
+    OC_TRY(big_rows(brows + hoff[13], n13, o));
+    fam("num_window", "numeric_window", true, brows + hoff[13], n13);
+    mk.mark("numeric_window");
+  }
   uint32_t n1 = 0;
   OC_TRY(readback(&n1, ovf, 1, s));
-  const uint32_t n13 = hoff[14] - hoff[13];
-  if (n13)
-    HIP_TRY(hipMemcpyAsync(ovf_rows + n1, brows + hoff[13], (size_t)n13 * 4,
+  const uint32_t nmove = inplace13 ? 0u : n13;
+  if (nmove)
+    HIP_TRY(hipMemcpyAsync(ovf_rows + n1, brows + hoff[13], (size_t)nmove * 4,
                            hipMemcpyDeviceToDevice, s));
-  const uint32_t novf = n1 + n13;
+  const uint32_t novf = n1 + nmove;
   h->overflow_rows = n1;
   uint64_t *src_off;
   OC_TRY(ar.get(&src_off, m + 1));
@@ -680,8 +691,8 @@ int Run::run_est() {
     unsigned long long *used;
     OC_TRY(ar.get(&used, 1));
     HIP_TRY(hipMemsetAsync(used, 0, 8, s));
-    OC_TRY(launch_assign_spare(ovf_rows, novf, ri.products, B.cols, h->total_est, used,
-                               src_off, s));
+    OC_TRY(launch_assign_spare(ovf_rows, novf, ri.products, ri.left, ri.right,
+                               h->total_est, used, src_off, s));
     unsigned long long hu = 0;
     OC_TRY(readback(&hu, used, 1, s));
     if (h->total_est + hu > cap_total) {
@@ -692,9 +703,10 @@ int Run::run_est() {
     NumOut o2 = o;
     o2.off = src_off;
     o2.cap = nullptr;
+    o2.nnz_hint = nullptr;  // the estimate was too small for these rows
     OC_TRY(big_rows(ovf_rows, novf, o2));
   }
-  fam("num_window", "numeric_overflow", true, ovf_rows, novf);
+  fam("num_overflow", "numeric_overflow", true, ovf_rows, novf);
   mk.mark("numeric_overflow");
   OC_TRY(alloc_c_from_counts(nnz));
   OC_TRY(launch_copy_rows(A.vtype, m, nullptr, nnz, src_off, tc, tv,
