@@ -125,6 +125,7 @@ typedef struct oc_config {
   int32_t tile_block;      /* workgroup size of the tiled kernel: 256/512/1024 */
   int32_t tile_products_cap; /* rank tiles: products per tile target (0 = none) */
   int32_t rank_layout;     /* rank tiles: products per lane 0 strided, 1 contiguous */
+  double learned_quantile; /* predictor: cumulative class probability to cover */
 } oc_config;
 
 /* Replaces: Config's default member initialisers (include/Utils.h:22-107). */
@@ -167,6 +168,52 @@ int oc_spgemm_set_workflow(oc_spgemm *h, int workflow);
  * tests feed the estimation workflow through it.  Rows whose estimate is
  * too small overflow and are recomputed exactly. */
 int oc_spgemm_set_estimates(oc_spgemm *h, const uint32_t *d_est);
+
+/* Value-only replays (no reference counterpart): for repeated products
+ * with a fixed structure (batches, iterative methods).  oc_spgemm_plan runs
+ * the precise workflow into dC and keeps its numeric plan (row information,
+ * numeric bins, scratch) in the handle; dC stays the caller's and must stay
+ * allocated while the plan is used.  oc_spgemm_replay recomputes dC's values
+ * from the current values of the same A and B buffers; oc_spgemm_capture
+ * records that replay into a HIP graph and oc_spgemm_graph_launch launches
+ * it (stream NULL: the handle's own stream, waited for). */
+int oc_spgemm_plan(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
+                   void *stream);
+int oc_spgemm_replay(oc_spgemm *h, void *stream);
+int oc_spgemm_capture(oc_spgemm *h);
+int oc_spgemm_graph_launch(oc_spgemm *h, void *stream);
+
+/* ------------------------------------------------------ batched SpGEMM */
+/* No reference counterpart.  count independent products C_i = A_i * B_i
+ * (B_i = A_i when Bs is NULL) run as one product of block-diagonal
+ * operands: the device inputs are concatenated (A_i's columns shifted by
+ * the earlier A_j.cols, B_i's by the earlier B_j.cols), so every launch of
+ * the path covers the rows of all products.  The concatenation needs fewer
+ * than 2^32 rows, columns and entries in each operand; A_i and B_i must
+ * share one value type and have 32-bit offsets. */
+typedef struct oc_batch oc_batch;
+oc_batch *oc_batch_create(const oc_config *cfg);
+void oc_batch_destroy(oc_batch *b);
+/* Copies the inputs into the block-diagonal operands (the inputs may be
+ * freed afterwards unless value refreshes via oc_batch_capture are used). */
+int oc_batch_set_inputs(oc_batch *b, const oc_csr *dAs, const oc_csr *dBs, uint32_t count,
+                        void *stream);
+/* Whole path (automatic workflow) on the concatenated operands. */
+int oc_batch_run(oc_batch *b, void *stream);
+/* Precise run that keeps the numeric plan, then a HIP graph that refreshes
+ * the concatenated values from the inputs' current values and recomputes
+ * every product (one graph launch per batch). */
+int oc_batch_plan(oc_batch *b, void *stream);
+int oc_batch_capture(oc_batch *b);
+int oc_batch_launch(oc_batch *b, void *stream);
+/* The batched result: a view of C (u64 offsets; product i is rows
+ * [row_base[i], row_base[i+1]) with columns shifted by col_base[i]) and
+ * host copies of the count+1 bases (may be NULL). */
+int oc_batch_result(const oc_batch *b, oc_csr *C, uint64_t *row_base, uint64_t *col_base);
+/* Product i as its own device CSR (u64 offsets, owned; oc_csr_free). */
+int oc_batch_extract(const oc_batch *b, uint32_t i, oc_csr *Ci, void *stream);
+/* The SpGEMM handle behind the batch (stats, timers, model). */
+oc_spgemm *oc_batch_handle(oc_batch *b);
 
 /* ---------------------------------------------------------- stage calls */
 /* gfx950 bin tables (re-derived for 160 KB LDS; see include/Common.h:78-291
@@ -278,6 +325,59 @@ int oc_csr_validate(const oc_csr *dM, uint64_t *bad_rows, void *stream);
 int oc_csr_row_set_hash(const oc_csr *dM, uint64_t *hash, void *stream);
 /* y = M x in FP64 (x, y device, cols / rows entries). */
 int oc_csr_spmv(const oc_csr *dM, const double *x, double *y, void *stream);
+
+/* ------------------------------------------ learned row-size predictor */
+/* No reference counterpart.  A 16 -> 64 -> 64 -> 8 MLP (BF16 weights on
+ * MFMA, FP32 accumulation) that predicts the class of each row's output
+ * count of C = A*B; classes end at the numeric hash capacities, 65536 and
+ * unbounded (oc_mlp_class_thresholds).  Parameters (OC_MLP_NPARAM floats)
+ * are laid out W1[16][64], b1[64], W2[64][64], b2[64], W3[64][8], b3[8].
+ * The predicted capacities replace the HyperLogLog estimates of the
+ * estimation workflow (kernels/SpGEMM.cuh:691-757); rows that exceed them
+ * overflow and are recomputed exactly. */
+#define OC_MLP_FEATURES 16
+#define OC_MLP_HIDDEN 64
+#define OC_MLP_CLASSES 8
+#define OC_MLP_NPARAM 5768
+typedef struct oc_mlp oc_mlp;
+/* He-normal weights from a seed, zero biases, Adam state cleared. */
+oc_mlp *oc_mlp_create(uint64_t seed);
+void oc_mlp_destroy(oc_mlp *m);
+int oc_mlp_get_params(const oc_mlp *m, float *host_params);
+int oc_mlp_set_params(oc_mlp *m, const float *host_params);
+/* Model file: 8-byte magic "OCMLP1", u32 parameter count, FP32 params. */
+int oc_mlp_save(const oc_mlp *m, const char *path);
+int oc_mlp_load(oc_mlp *m, const char *path);
+/* Class upper bounds (OC_MLP_CLASSES entries, the last 0xFFFFFFFF). */
+int oc_mlp_class_thresholds(uint32_t *thr);
+/* Features (device out, A.rows x 16 floats) of the rows of C = A*B. */
+int oc_mlp_features(const oc_csr *dA, const oc_csr *dB, float *feat, void *stream);
+/* Class labels (device u8) of exact per-row counts (device u32). */
+int oc_mlp_labels(const uint32_t *nnz, uint64_t n, uint8_t *labels, void *stream);
+/* Logits (device out, n x 8 floats). */
+int oc_mlp_forward(const oc_mlp *m, const float *feat, uint64_t n, float *logits,
+                   void *stream);
+/* Gradients of the mean cross-entropy of one minibatch (n <= 65536) into
+ * the model's gradient buffer (read with oc_mlp_get_grads). */
+int oc_mlp_backward(oc_mlp *m, const float *feat, const uint8_t *labels, uint64_t n,
+                    void *stream);
+int oc_mlp_get_grads(const oc_mlp *m, float *host_grads, float *host_loss);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) with the last gradients. */
+int oc_mlp_step(oc_mlp *m, float lr, void *stream);
+/* Minibatch training over device data in order; mean loss per epoch
+ * (host out, may be NULL). */
+int oc_mlp_train(oc_mlp *m, const float *feat, const uint8_t *labels, uint64_t n,
+                 uint32_t batch, uint32_t epochs, float lr, float *epoch_loss,
+                 void *stream);
+/* Per-row capacities (device out, A.rows) for C = A*B: the smallest class
+ * whose cumulative probability reaches `quantile`, raised to the longest
+ * referenced B row and clipped to min(products, column span). */
+int oc_mlp_estimates(const oc_mlp *m, const oc_csr *dA, const oc_csr *dB,
+                     double quantile, uint32_t *est, void *stream);
+/* Use a trained predictor in oc_spgemm_run (NULL: back to HyperLogLog):
+ * on the HyperLogLog-sampled path the estimation workflow then runs on the
+ * predicted capacities.  The model must outlive the handle's runs. */
+int oc_spgemm_set_model(oc_spgemm *h, const oc_mlp *m, double quantile);
 
 /* ------------------------------------------------------ device memory */
 /* Thin HIP helpers for bindings without their own device allocator (no

@@ -1,0 +1,158 @@
+"""CPU oracle of the learned row-size predictor (test infrastructure only).
+
+FP64 numpy restatement of the 16 -> 64 -> 64 -> 8 MLP of
+research_amd/csrc/k_mlp.hip: features, class labels, forward pass,
+softmax cross-entropy gradients and the Adam update.  There is no
+reference counterpart (the predictor is an addition to the reference's
+method); the arithmetic is pinned by finite differences of the exact
+(no-rounding) gradients, by BF16 rounding known answers, and by
+brute-force feature values on small matrices.
+
+`rnd` hooks apply BF16 round-to-nearest-even at the points where the GPU
+kernels round (inputs, hidden activations, dZ), so the emulated pass
+differs from the GPU only by FP32 accumulation order.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import oracle as ora
+
+F, H, K = 16, 64, 8
+OFF = {"W1": 0, "b1": F * H, "W2": F * H + H, "b2": F * H + H + H * H,
+       "W3": F * H + 2 * H + H * H, "b3": F * H + 2 * H + H * H + H * K}
+NPARAM = OFF["b3"] + K  # 5768
+
+
+def bf16(x) -> np.ndarray:
+    """Round to BF16 (nearest even) through float32, returned as float64."""
+    f = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def f32(x) -> np.ndarray:
+    return np.asarray(x, dtype=np.float32).astype(np.float64)
+
+
+def ident(x) -> np.ndarray:
+    return np.asarray(x, dtype=np.float64)
+
+
+def unpack(p: np.ndarray):
+    p = np.asarray(p, dtype=np.float64)
+    return (p[OFF["W1"]:OFF["b1"]].reshape(F, H), p[OFF["b1"]:OFF["W2"]],
+            p[OFF["W2"]:OFF["b2"]].reshape(H, H), p[OFF["b2"]:OFF["W3"]],
+            p[OFF["W3"]:OFF["b3"]].reshape(H, K), p[OFF["b3"]:NPARAM])
+
+
+def pack(W1, b1, W2, b2, W3, b3) -> np.ndarray:
+    return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, W3.ravel(), b3])
+
+
+def forward(p, X, rnd=bf16):
+    """Logits and the intermediates (rnd = bf16 emulates the GPU)."""
+    W1, b1, W2, b2, W3, b3 = unpack(p)
+    r = rnd if rnd is not ident else ident
+    post = f32 if rnd is not ident else ident  # FP32 accumulator before rounding
+    Xb = r(X)
+    W1b, W2b, W3b = r(W1), r(W2), r(W3)
+    Z1 = post(Xb @ W1b + b1)
+    H1 = r(np.where(Z1 > 0, Z1, 0.0))
+    Z2 = post(H1 @ W2b + b2)
+    H2 = r(np.where(Z2 > 0, Z2, 0.0))
+    Z3 = post(H2 @ W3b + b3)
+    return Z3, (Xb, W1b, W2b, W3b, H1, H2)
+
+
+def softmax(z):
+    m = z.max(axis=1, keepdims=True)
+    e = np.exp(z - m)
+    return e / e.sum(axis=1, keepdims=True)
+
+
+def loss_and_grads(p, X, y, rnd=bf16):
+    """Mean softmax cross-entropy over the rows and its gradient (NPARAM)."""
+    n = X.shape[0]
+    Z3, (Xb, W1b, W2b, W3b, H1, H2) = forward(p, X, rnd)
+    P = softmax(Z3)
+    loss = float(-np.log(P[np.arange(n), y]).mean())
+    onehot = np.zeros_like(P)
+    onehot[np.arange(n), y] = 1.0
+    r = rnd
+    post = f32 if rnd is not ident else ident
+    dZ3 = r(post((P - onehot) * post(1.0 / n)))
+    dZ2 = r(np.where(H2 != 0, post(dZ3 @ W3b.T), 0.0))
+    dZ1 = r(np.where(H1 != 0, post(dZ2 @ W2b.T), 0.0))
+    g = pack(Xb.T @ dZ1, dZ1.sum(0), H1.T @ dZ2, dZ2.sum(0), H2.T @ dZ3, dZ3.sum(0))
+    return loss, g
+
+
+def adam(p, m1, m2, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (t counts from 1); returns the new (p, m1, m2)."""
+    m1 = b1 * m1 + (1 - b1) * g
+    m2 = b2 * m2 + (1 - b2) * g * g
+    p = p - lr * (m1 / (1 - b1 ** t)) / (np.sqrt(m2 / (1 - b2 ** t)) + eps)
+    return p, m1, m2
+
+
+# ------------------------------------------------------------- features
+def features(A, B) -> np.ndarray:
+    """The sixteen per-row features of C = A*B (FP64)."""
+    an = ora.analysis(A, B)
+    m = A.rows
+    P = an["products"].astype(np.float64)
+    nA = np.diff(np.asarray(A.row_offsets, dtype=np.int64)).astype(np.float64)
+    mb = an["max_b"].astype(np.float64)
+    ab = an["avg_b"].astype(np.float64)
+    lo = an["left"].astype(np.int64)
+    hi = an["right"].astype(np.int64)
+    span = np.where(hi >= lo, hi - lo + 1, 0).astype(np.float64)
+    ub = np.minimum(P, span)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        uni = np.where(span > 0, span * (1 - np.exp(-P / np.where(span > 0, span, 1))), 0.0)
+        bal = np.where((nA > 0) & (mb > 0), P / np.where((nA > 0) & (mb > 0), nA * mb, 1), 0.0)
+    lg = lambda v: np.log2(1.0 + v) / 16.0  # noqa: E731
+    b_nnz = len(B.col_ids)
+    out = np.zeros((m, F))
+    out[:, 0] = lg(P)
+    out[:, 1] = lg(nA)
+    out[:, 2] = lg(mb)
+    out[:, 3] = lg(ab)
+    out[:, 4] = lg(span)
+    out[:, 5] = lg(ub)
+    out[:, 6] = lg(uni)
+    out[:, 7] = lg(P) - lg(mb)
+    out[:, 8] = bal
+    out[:, 9] = span / B.cols if B.cols else 0.0
+    out[:, 10] = np.where(span > 0, lo / B.cols, 0.0) if B.cols else 0.0
+    out[:, 11] = np.arange(m) / m
+    out[:, 12] = np.log2(1.0 + (b_nnz / B.rows if B.rows else 0.0)) / 16.0
+    out[:, 13] = np.log2(1.0 + len(A.col_ids) / m) / 16.0
+    out[:, 14] = np.log2(1.0 + B.cols) / 16.0
+    out[:, 15] = 1.0
+    return out
+
+
+def labels(nnz, thresholds) -> np.ndarray:
+    """Class = first c with nnz <= thresholds[c] (the last class is open)."""
+    t = np.asarray(thresholds[:K - 1], dtype=np.int64)
+    return np.searchsorted(t, np.asarray(nnz, dtype=np.int64), side="left").astype(np.uint8)
+
+
+def capacities(logits, thresholds, products, max_b, left, right, q) -> np.ndarray:
+    """Per-row capacity from logits: smallest class with cumulative
+    probability >= q, its threshold (min(products, span) for the last
+    class), raised to max_b and clipped to min(products, span)."""
+    P = softmax(np.asarray(logits, dtype=np.float64))
+    cum = np.cumsum(P, axis=1)
+    cls = np.minimum((cum < q).sum(axis=1), K - 1)
+    lo = np.asarray(left, dtype=np.int64)
+    hi = np.asarray(right, dtype=np.int64)
+    span = np.where(hi >= lo, hi - lo + 1, 0)
+    ub = np.minimum(np.asarray(products, dtype=np.int64), span)
+    thr = np.asarray(list(thresholds[:K - 1]) + [0], dtype=np.int64)
+    e = np.where(cls < K - 1, thr[cls], ub)
+    e = np.maximum(e, np.asarray(max_b, dtype=np.int64))
+    return np.minimum(e, ub).astype(np.uint32), cls

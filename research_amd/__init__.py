@@ -6,10 +6,12 @@ orchestrator behind the C ABI of ``include/ocean_capi.h``) and the
 Python callers and tests.
 """
 from .ocean import (  # noqa: F401
+    Batch,
     CSR,
     Config,
     DeviceCSR,
     DeviceBuffer,
+    Mlp,
     OceanError,
     SpGEMM,
     lib,

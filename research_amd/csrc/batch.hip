@@ -1,0 +1,319 @@
+// Batched SpGEMM: many independent products C_i = A_i * B_i as one product
+// of block-diagonal operands.
+//
+// The inputs are concatenated on the device: A_i's rows follow A_{i-1}'s,
+// its column ids are shifted by the sum of the earlier A_j.cols (which is
+// B_i's row offset in the concatenated B), and B_i's column ids by the sum
+// of the earlier B_j.cols.  The product of the concatenated operands is
+// block diagonal with C_i in its blocks, so one run of the whole path (one
+// analysis, one binning over the rows of every product, one launch per bin)
+// computes every C_i; launch counts do not grow with the batch size.
+//
+// For repeated batches with a fixed structure the batch keeps a numeric
+// plan, and one captured HIP graph refreshes the concatenated values from
+// the inputs and recomputes every product's values.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dev_common.hpp"
+#include "launch.hpp"
+
+namespace oc {
+
+#define HIP_TRY(call)                                                         \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                    \
+                hipGetErrorString(e_));                                       \
+      return e_ == hipErrorOutOfMemory ? OC_ERR_NOMEM : OC_ERR_HIP;           \
+    }                                                                         \
+  } while (0)
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+struct BatchDesc {
+  const uint32_t *rp, *ci;
+  const void *v;
+  uint64_t rows, nnz, row_base, nnz_base;
+  uint32_t col_base, pad;
+};
+
+// One workgroup per input matrix.  VALUES_ONLY: refresh the values.
+template <bool VALUES_ONLY>
+__global__ __launch_bounds__(256) void k_batch_concat(const BatchDesc *__restrict__ d,
+                                                      uint32_t count, int vbytes,
+                                                      uint32_t *__restrict__ rp_out,
+                                                      uint32_t *__restrict__ ci_out,
+                                                      uint8_t *__restrict__ v_out,
+                                                      uint64_t total_rows, uint64_t total_nnz) {
+  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+    const BatchDesc x = d[i];
+    const uint32_t r0 = x.rp[0];
+    if (!VALUES_ONLY) {
+      for (uint64_t j = threadIdx.x; j < x.rows; j += blockDim.x)
+        rp_out[x.row_base + j] = (uint32_t)(x.nnz_base + (x.rp[j] - r0));
+      for (uint64_t e = threadIdx.x; e < x.nnz; e += blockDim.x)
+        ci_out[x.nnz_base + e] = x.ci[r0 + e] + x.col_base;
+    }
+    if (vbytes == 8) {
+      const double *src = (const double *)x.v + r0;
+      double *dst = (double *)v_out + x.nnz_base;
+      for (uint64_t e = threadIdx.x; e < x.nnz; e += blockDim.x) dst[e] = src[e];
+    } else {
+      const float *src = (const float *)x.v + r0;
+      float *dst = (float *)v_out + x.nnz_base;
+      for (uint64_t e = threadIdx.x; e < x.nnz; e += blockDim.x) dst[e] = src[e];
+    }
+  }
+  if (!VALUES_ONLY && blockIdx.x == 0 && threadIdx.x == 0)
+    rp_out[total_rows] = (uint32_t)total_nnz;
+}
+
+// Product i of a batched result as its own CSR (u64 offsets from 0, column
+// ids rebased).
+template <typename T>
+__global__ void k_batch_extract(const uint64_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+                                const T *__restrict__ v, uint64_t row_base, uint64_t rows,
+                                uint32_t col_base, uint64_t *__restrict__ rp_out,
+                                uint32_t *__restrict__ ci_out, T *__restrict__ v_out) {
+  const uint64_t base = rp[row_base];
+  const uint64_t nnz = rp[row_base + rows] - base;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  for (uint64_t j = t0; j <= rows; j += stride) rp_out[j] = rp[row_base + j] - base;
+  for (uint64_t e = t0; e < nnz; e += stride) {
+    ci_out[e] = ci[base + e] - col_base;
+    v_out[e] = v[base + e];
+  }
+}
+
+int spgemm_replay_async(oc_spgemm *h, hipStream_t s);
+void spgemm_drop_plan(oc_spgemm *h);
+
+}  // namespace oc
+
+using namespace oc;
+
+struct oc_batch {
+  oc_spgemm *h = nullptr;
+  uint32_t count = 0;
+  int vtype = OC_F64;
+  bool b_is_a = false;
+  oc_csr A{}, B{}, C{};
+  std::vector<uint64_t> row_base, col_base;  // count + 1 each
+  uint64_t a_total_nnz = 0, b_total_nnz = 0;
+  BatchDesc *dA = nullptr, *dB = nullptr;
+  bool planned = false;
+  hipStream_t own = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  void drop_result() {
+    drop_graph();
+    spgemm_drop_plan(h);  // the plan refers to C
+    oc_csr_free(&C);
+    planned = false;
+  }
+  void drop_inputs() {
+    drop_result();
+    if (!b_is_a) oc_csr_free(&B);
+    std::memset(&B, 0, sizeof(B));
+    oc_csr_free(&A);
+    if (dA) (void)hipFree(dA);
+    if (dB) (void)hipFree(dB);
+    dA = dB = nullptr;
+    count = 0;
+  }
+};
+
+static int concat(const oc_batch *b, const BatchDesc *d, const oc_csr &M, uint64_t total_nnz,
+                  bool values_only, hipStream_t s) {
+  const int vb = (int)vsize(b->vtype);
+  const uint32_t grid = std::min<uint32_t>(b->count, 8192);
+  if (values_only)
+    hipLaunchKernelGGL(k_batch_concat<true>, dim3(grid), dim3(256), 0, s, d, b->count, vb,
+                       (uint32_t *)M.row_offsets, M.col_ids, (uint8_t *)M.values, M.rows,
+                       total_nnz);
+  else
+    hipLaunchKernelGGL(k_batch_concat<false>, dim3(grid), dim3(256), 0, s, d, b->count, vb,
+                       (uint32_t *)M.row_offsets, M.col_ids, (uint8_t *)M.values, M.rows,
+                       total_nnz);
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
+extern "C" oc_batch *oc_batch_create(const oc_config *cfg) {
+  oc_batch *b = new (std::nothrow) oc_batch();
+  if (!b) {
+    set_error("out of host memory");
+    return nullptr;
+  }
+  b->h = oc_spgemm_create(cfg);
+  if (!b->h) {
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
+extern "C" void oc_batch_destroy(oc_batch *b) {
+  if (!b) return;
+  b->drop_inputs();
+  if (b->own) (void)hipStreamDestroy(b->own);
+  oc_spgemm_destroy(b->h);
+  delete b;
+}
+
+extern "C" int oc_batch_set_inputs(oc_batch *b, const oc_csr *As, const oc_csr *Bs,
+                                   uint32_t count, void *stream) {
+  OC_REQUIRE(b && As && count > 0, "oc_batch_set_inputs: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  b->drop_inputs();
+  const int vt = As[0].vtype;
+  OC_REQUIRE(vt == OC_F64 || vt == OC_F32, "bad value type");
+  std::vector<BatchDesc> da(count), db(count);
+  uint64_t rows = 0, acols = 0, bcols = 0, annz = 0, bnnz = 0;
+  std::vector<uint64_t> rb(count + 1), cb(count + 1);
+  for (uint32_t i = 0; i < count; ++i) {
+    const oc_csr &a = As[i];
+    const oc_csr &bb = Bs ? Bs[i] : As[i];
+    OC_REQUIRE(a.on_device && bb.on_device && a.off_bytes == 4 && bb.off_bytes == 4,
+               "batch input %u: need device matrices with 32-bit offsets", i);
+    OC_REQUIRE(a.vtype == vt && bb.vtype == vt, "batch input %u: value types differ", i);
+    OC_REQUIRE(a.cols == bb.rows, "batch input %u: Incompatible matrix dimensions!", i);
+    da[i] = BatchDesc{(const uint32_t *)a.row_offsets, a.col_ids, a.values, a.rows, a.nnz,
+                      rows, annz, (uint32_t)acols, 0};
+    db[i] = BatchDesc{(const uint32_t *)bb.row_offsets, bb.col_ids, bb.values, bb.rows, bb.nnz,
+                      acols, bnnz, (uint32_t)bcols, 0};
+    rb[i] = rows;
+    cb[i] = bcols;
+    rows += a.rows;
+    acols += a.cols;
+    bcols += bb.cols;
+    annz += a.nnz;
+    bnnz += bb.nnz;
+    OC_REQUIRE(rows < (1ull << 32) && acols < (1ull << 32) && bcols < (1ull << 32) &&
+                   annz < (1ull << 32) && bnnz < (1ull << 32),
+               "batch exceeds 32-bit rows, columns or entries at input %u", i);
+  }
+  rb[count] = rows;
+  cb[count] = bcols;
+  b->count = count;
+  b->vtype = vt;
+  b->b_is_a = Bs == nullptr;
+  b->row_base = rb;
+  b->col_base = cb;
+  b->a_total_nnz = annz;
+  b->b_total_nnz = bnnz;
+  OC_TRY(oc_csr_alloc(&b->A, rows, acols, annz, vt, 4, 1));
+  HIP_TRY(hipMalloc((void **)&b->dA, sizeof(BatchDesc) * count));
+  HIP_TRY(hipMemcpy(b->dA, da.data(), sizeof(BatchDesc) * count, hipMemcpyHostToDevice));
+  OC_TRY(concat(b, b->dA, b->A, annz, false, s));
+  if (b->b_is_a) {
+    b->B = b->A;
+    b->B.owned = 0;
+  } else {
+    OC_TRY(oc_csr_alloc(&b->B, acols, bcols, bnnz, vt, 4, 1));
+    HIP_TRY(hipMalloc((void **)&b->dB, sizeof(BatchDesc) * count));
+    HIP_TRY(hipMemcpy(b->dB, db.data(), sizeof(BatchDesc) * count, hipMemcpyHostToDevice));
+    OC_TRY(concat(b, b->dB, b->B, bnnz, false, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  return OC_OK;
+}
+
+extern "C" int oc_batch_run(oc_batch *b, void *stream) {
+  OC_REQUIRE(b && b->count, "oc_batch_run: no inputs");
+  b->drop_result();
+  return oc_spgemm_run(b->h, &b->A, &b->B, &b->C, stream);
+}
+
+extern "C" int oc_batch_plan(oc_batch *b, void *stream) {
+  OC_REQUIRE(b && b->count, "oc_batch_plan: no inputs");
+  b->drop_result();
+  OC_TRY(oc_spgemm_plan(b->h, &b->A, &b->B, &b->C, stream));
+  b->planned = true;
+  return OC_OK;
+}
+
+extern "C" int oc_batch_capture(oc_batch *b) {
+  OC_REQUIRE(b && b->planned, "oc_batch_capture: run oc_batch_plan first");
+  b->drop_graph();
+  if (!b->own) HIP_TRY(hipStreamCreate(&b->own));
+  HIP_TRY(hipStreamBeginCapture(b->own, hipStreamCaptureModeRelaxed));
+  int rc = concat(b, b->dA, b->A, b->a_total_nnz, true, b->own);
+  if (rc == OC_OK && !b->b_is_a) rc = concat(b, b->dB, b->B, b->b_total_nnz, true, b->own);
+  if (rc == OC_OK) rc = spgemm_replay_async(b->h, b->own);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->own, &g);
+  if (rc != OC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_TRY(e);
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return OC_OK;
+}
+
+extern "C" int oc_batch_launch(oc_batch *b, void *stream) {
+  OC_REQUIRE(b && b->exec, "oc_batch_launch: run oc_batch_capture first");
+  if (stream) {
+    HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  } else {
+    HIP_TRY(hipGraphLaunch(b->exec, b->own));
+    HIP_TRY(hipStreamSynchronize(b->own));
+  }
+  return OC_OK;
+}
+
+extern "C" int oc_batch_result(const oc_batch *b, oc_csr *C, uint64_t *row_base,
+                               uint64_t *col_base) {
+  OC_REQUIRE(b && C && b->C.row_offsets, "oc_batch_result: no result");
+  *C = b->C;
+  C->owned = 0;  // a view: the batch keeps the memory
+  if (row_base) std::memcpy(row_base, b->row_base.data(), 8 * (b->count + 1));
+  if (col_base) std::memcpy(col_base, b->col_base.data(), 8 * (b->count + 1));
+  return OC_OK;
+}
+
+extern "C" int oc_batch_extract(const oc_batch *b, uint32_t i, oc_csr *Ci, void *stream) {
+  OC_REQUIRE(b && Ci && b->C.row_offsets, "oc_batch_extract: no result");
+  OC_REQUIRE(i < b->count, "oc_batch_extract: product %u of %u", i, b->count);
+  hipStream_t s = (hipStream_t)stream;
+  const uint64_t r0 = b->row_base[i], rows = b->row_base[i + 1] - r0;
+  uint64_t off[2];
+  HIP_TRY(hipMemcpyAsync(&off[0], (const uint64_t *)b->C.row_offsets + r0, 8,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&off[1], (const uint64_t *)b->C.row_offsets + r0 + rows, 8,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const uint64_t nnz = off[1] - off[0];
+  OC_TRY(oc_csr_alloc(Ci, rows, b->col_base[i + 1] - b->col_base[i], nnz, b->vtype, 8, 1));
+  const uint64_t work = std::max<uint64_t>(rows + 1, nnz);
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + 255) / 256, 4096));
+  if (b->vtype == OC_F64)
+    hipLaunchKernelGGL(k_batch_extract<double>, dim3(grid), dim3(256), 0, s,
+                       (const uint64_t *)b->C.row_offsets, b->C.col_ids,
+                       (const double *)b->C.values, r0, rows, (uint32_t)b->col_base[i],
+                       (uint64_t *)Ci->row_offsets, Ci->col_ids, (double *)Ci->values);
+  else
+    hipLaunchKernelGGL(k_batch_extract<float>, dim3(grid), dim3(256), 0, s,
+                       (const uint64_t *)b->C.row_offsets, b->C.col_ids,
+                       (const float *)b->C.values, r0, rows, (uint32_t)b->col_base[i],
+                       (uint64_t *)Ci->row_offsets, Ci->col_ids, (float *)Ci->values);
+  LAUNCH_CHECK();
+  HIP_TRY(hipStreamSynchronize(s));
+  return OC_OK;
+}
+
+extern "C" oc_spgemm *oc_batch_handle(oc_batch *b) { return b ? b->h : nullptr; }

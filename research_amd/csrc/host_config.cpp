@@ -63,6 +63,7 @@ extern "C" void oc_config_default(oc_config *c) {
   c->tile_block = 1024;
   c->tile_products_cap = 0;
   c->rank_layout = 0;
+  c->learned_quantile = 0.99;
 }
 
 namespace {
@@ -278,6 +279,7 @@ extern "C" int oc_config_from_json(oc_config *c, const char *text) {
     I("tile_block", tile_block)
     I("tile_products_cap", tile_products_cap)
     I("rank_layout", rank_layout)
+    D("learned_quantile", learned_quantile)
 #undef B
 #undef I
 #undef D

@@ -688,6 +688,17 @@ static int set_lds(K kernel, size_t bytes) {
 // Column tiles of W = 2^logW columns.  ts[k * T1 + t] is the index of the
 // first entry of B row k whose column is >= t * W (T1 = tiles + 1), so the
 // part of any B row inside any tile is known without walking the row.
+// Clears work counters with a kernel rather than a memset node, so the
+// launch sequence can be captured into a HIP graph and replayed.
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+static int zero_u32(uint32_t *p, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p, n);
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
 __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t *__restrict__ b_rp,
                                                      const uint32_t *__restrict__ b_ci,
                                                      uint32_t rows, uint32_t logW,
@@ -1149,7 +1160,7 @@ int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint
   const uint32_t cap = (uint32_t)(((t.lds_max_block - fixed) / 12) & ~63ull);
   const size_t lds = fixed + (size_t)cap * 12;
   uint32_t *counter = (uint32_t *)scratch;
-  HIP_TRY(hipMemsetAsync(counter, 0, 4, s));
+  OC_TRY(zero_u32(counter, 1, s));
   const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu));
 #define GO(T, C)                                                                    \
   do {                                                                              \
@@ -1201,7 +1212,7 @@ int launch_num_tiles(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uin
   const Tables &t = tables();
   const size_t lds = tile_lds(1u << logW, bs);
   uint32_t *counter = (uint32_t *)scratch;
-  HIP_TRY(hipMemsetAsync(counter, 0, 4, s));
+  OC_TRY(zero_u32(counter, 1, s));
   // persistent: as many workgroups as fit on the chip at once
   const uint32_t per_cu = std::max<uint32_t>(
       1, std::min<uint32_t>(2048 / bs, (uint32_t)(t.lds_per_cu / lds)));
@@ -1254,7 +1265,7 @@ static int window_go(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
   OC_TRY(set_lds(k_window<T, SYM>, lds));
   uint32_t *counter = (uint32_t *)scratch;
   uint32_t *cur = counter + 64;
-  HIP_TRY(hipMemsetAsync(counter, 0, 4, s));
+  OC_TRY(zero_u32(counter, 1, s));
   const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu));
   KOut<T> ko = kout<T>(o);
   hipLaunchKernelGGL((k_window<T, SYM>), dim3(grid), dim3(kWinBlock), lds, s,

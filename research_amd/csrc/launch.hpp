@@ -40,6 +40,13 @@ struct RowInfo {
   uint32_t *products, *max_b, *avg_b, *left, *right;
 };
 
+// Learned row-size predictor (k_mlp.hip): features of the rows of C = A*B
+// and per-row capacities from the model.
+int launch_mlp_features(const DevCsr &A, const DevCsr &B, const RowInfo &ri, float *feat,
+                        hipStream_t s);
+int launch_mlp_estimates(const oc_mlp *m, const float *feat, uint64_t n, const RowInfo &ri,
+                         double q, uint32_t *est, hipStream_t s);
+
 int launch_analysis(const DevCsr &A, const DevCsr &B, const RowInfo &ri,
                     uint32_t *max_a_nnz, uint32_t *max_products,
                     hipStream_t s);

@@ -69,11 +69,13 @@ struct Arena {
 struct Marks {
   hipStream_t s;
   std::vector<std::pair<std::string, hipEvent_t>> ev;
+  bool enabled = true;
   explicit Marks(hipStream_t st) : s(st) {}
   ~Marks() {
     for (auto &p : ev) (void)hipEventDestroy(p.second);
   }
   void mark(const char *name) {
+    if (!enabled) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
     (void)hipEventRecord(e, s);
@@ -138,6 +140,42 @@ struct oc_spgemm {
   // estimation override (learned predictor / forced fallback tests)
   const uint32_t *est_override = nullptr;
   int force_workflow = -1;  // -1 none, 0 precise, 1 estimation
+  // learned row-size predictor (set by the caller or loaded from the config)
+  const oc_mlp *model = nullptr;
+  oc_mlp *owned_model = nullptr;
+  double model_q = 0.99;
+  bool learned_run = false;
+  // Numeric plan of a precise run kept for value-only replays: the row
+  // information, numeric bins and scratch stay allocated; C is the caller's.
+  struct Plan {
+    bool valid = false;
+    std::vector<void *> mem;
+    DevCsr A{}, B{};
+    RowInfo ri{};
+    const uint32_t *brows = nullptr;
+    uint32_t hoff[kBinSlots + 1] = {0};
+    NumOut o{};
+    uint32_t *tile_ts = nullptr;
+    uint32_t tile_logw = 0, tile_t1 = 0, max_a_nnz = 0;
+    int tiles_ok = -1;
+    void *win_scratch = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } plan;
+  bool keep_plan = false;
+  hipStream_t own_stream = nullptr;  // capture needs a non-default stream
+  void drop_plan() {
+    if (plan.exec) (void)hipGraphExecDestroy(plan.exec);
+    if (plan.graph) (void)hipGraphDestroy(plan.graph);
+    if (!plan.mem.empty()) (void)hipDeviceSynchronize();
+    for (void *p : plan.mem) (void)hipFree(p);
+    plan = Plan();
+  }
+  ~oc_spgemm() {
+    drop_plan();
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+    oc_mlp_destroy(owned_model);
+  }
 };
 
 namespace {
@@ -252,9 +290,10 @@ struct Run {
     uint32_t n;
   };
   std::vector<Fam> fams;
+  bool replay = false;  // value-only replay of a kept plan: no marks, no accounting
   void fam(const char *name, const char *stage, bool numeric, const uint32_t *rows,
            uint32_t n) {
-    if (c.track_stage_times && n) fams.push_back(Fam{name, stage, numeric, rows, n});
+    if (c.track_stage_times && n && !replay) fams.push_back(Fam{name, stage, numeric, rows, n});
   }
   Run(oc_spgemm *hh, hipStream_t st, oc_csr *cc)
       : h(hh), c(hh->cfg), s(st), ar(st), mk(st), C(cc) {}
@@ -604,6 +643,22 @@ int Run::run_precise() {
   OC_TRY(readback(&n, ovf + 1, 1, s));
   h->precise_recount = n;
   OC_TRY(big_rows(ovf_rows, n, o));
+  if (h->keep_plan) {
+    oc_spgemm::Plan &p = h->plan;
+    p.A = A;
+    p.B = B;
+    p.ri = ri;
+    p.brows = brows2;
+    std::memcpy(p.hoff, hoff, sizeof(hoff));
+    p.o = o;
+    p.tile_ts = tile_ts;
+    p.tile_logw = tile_logw;
+    p.tile_t1 = tile_t1;
+    p.tiles_ok = tiles_ok;
+    p.win_scratch = win_scratch;
+    p.max_a_nnz = h->max_a_nnz;
+    p.valid = n == 0;  // a recount would make the numeric phase two-step
+  }
   return OC_OK;
 }
 
@@ -619,6 +674,11 @@ int Run::run_est() {
   mk.mark("est_malloc");
   if (h->est_override) {
     HIP_TRY(hipMemcpyAsync(est, h->est_override, m * 4, hipMemcpyDeviceToDevice, s));
+  } else if (h->learned_run) {
+    float *feat;
+    OC_TRY(ar.get(&feat, m * OC_MLP_FEATURES));
+    OC_TRY(launch_mlp_features(A, B, ri, feat, s));
+    OC_TRY(launch_mlp_estimates(h->model, feat, m, ri, h->model_q, est, s));
   } else {
     uint8_t *tmp;
     OC_TRY(ar.get(&tmp, hll_merge_temp_bytes(m)));
@@ -802,6 +862,7 @@ std::string stats_json(const oc_spgemm *h) {
     j.boolean("dense_bitmap_query", d.dense_query_bitmap);
   }
   if (workflow == 3) j.u64("fat_runs_precise_kernels", 1);
+  j.boolean("learned_estimates", h->learned_run);
   j.close();
   if (workflow <= 1) {
     j.open("sampling");
@@ -980,14 +1041,23 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
     h->d = Decision();
   } else {
     rc = r.prologue_and_analysis();
-    if (rc == OC_OK && h->d.ana1_type == 1) rc = r.sampling();
-    if (rc == OC_OK && h->force_workflow >= 0 && h->d.ana1_type == 1)
-      h->d.ana2_type = h->force_workflow;
-    if (rc == OC_OK) {
-      const Decision &d = h->d;
-      if (d.ana1_type == 0) rc = r.run_usparse();
-      else if (d.ana1_type == 2 || d.ana2_type == 0) rc = r.run_precise();
-      else rc = r.run_est();
+    h->learned_run = false;
+    if (rc == OC_OK && h->keep_plan) {
+      h->d.ana1_type = 1;  // reported as the precise workflow
+      h->d.ana2_type = 0;
+      rc = r.run_precise();
+    } else if (rc == OC_OK) {
+      h->learned_run = h->model != nullptr && h->d.ana1_type == 1 && h->force_workflow != 0;
+      if (h->learned_run) h->d.ana2_type = 1;  // predicted capacities
+      else if (h->d.ana1_type == 1) rc = r.sampling();
+      if (rc == OC_OK && h->force_workflow >= 0 && h->d.ana1_type == 1)
+        h->d.ana2_type = h->force_workflow;
+      if (rc == OC_OK) {
+        const Decision &d = h->d;
+        if (d.ana1_type == 0) rc = r.run_usparse();
+        else if (d.ana1_type == 2 || d.ana2_type == 0) rc = r.run_precise();
+        else rc = r.run_est();
+      }
     }
   }
   r.mk.mark("end");
@@ -1018,6 +1088,12 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
     rc = OC_ERR_HIP;
   }
   r.mk.resolve(h->stage_ms, h->total_ms);
+  if (rc == OC_OK && h->keep_plan && h->plan.valid) {
+    h->plan.mem = r.ar.ptrs;  // kept for replays, freed by drop_plan
+    r.ar.ptrs.clear();
+  } else if (h->keep_plan) {
+    h->plan.valid = false;
+  }
   r.ar.release();
   if (rc != OC_OK) {
     oc_csr_free(dC);
@@ -1029,7 +1105,43 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
   return OC_OK;
 }
 
+// Numeric phase of a kept plan, issued on stream s (no allocation, no host
+// synchronisation, no memset nodes: safe to capture into a graph).
+int replay_impl(oc_spgemm *h, hipStream_t s) {
+  oc_spgemm::Plan &p = h->plan;
+  OC_REQUIRE(p.valid, "no numeric plan: run oc_spgemm_plan first");
+  Run r(h, s, nullptr);
+  r.replay = true;
+  r.mk.enabled = false;
+  r.A = p.A;
+  r.B = p.B;
+  r.ri = p.ri;
+  r.tile_ts = p.tile_ts;
+  r.tile_logw = p.tile_logw;
+  r.tile_t1 = p.tile_t1;
+  r.tiles_ok = p.tiles_ok;
+  r.win_scratch = p.win_scratch;
+  const uint32_t keep_max = h->max_a_nnz;
+  h->max_a_nnz = p.max_a_nnz;
+  int rc = r.numeric_bins(p.brows, p.hoff, p.o, true);
+  h->max_a_nnz = keep_max;
+  if (rc == OC_OK && !r.ar.ptrs.empty()) {
+    set_error("internal: a replay allocated memory");
+    rc = OC_ERR_INVALID;
+  }
+  return rc;
+}
+
 }  // namespace
+
+namespace oc {
+// Used by the batched layer (batch.hip) to record the numeric phase into
+// its own graph after refreshing the concatenated values.
+int spgemm_replay_async(oc_spgemm *h, hipStream_t s) { return replay_impl(h, s); }
+void spgemm_drop_plan(oc_spgemm *h) {
+  if (h) h->drop_plan();
+}
+}  // namespace oc
 
 // ================================================================== C ABI
 extern "C" oc_spgemm *oc_spgemm_create(const oc_config *cfg) {
@@ -1040,6 +1152,16 @@ extern "C" oc_spgemm *oc_spgemm_create(const oc_config *cfg) {
   }
   if (cfg) h->cfg = *cfg;
   else oc_config_default(&h->cfg);
+  if (h->cfg.use_learned_binning && h->cfg.learned_model_file[0]) {
+    h->owned_model = oc_mlp_create(0);
+    if (!h->owned_model || oc_mlp_load(h->owned_model, h->cfg.learned_model_file) != OC_OK) {
+      delete h;  // the error message of the failing call stays set
+      return nullptr;
+    }
+    h->model = h->owned_model;
+    const double q = h->cfg.learned_quantile;
+    h->model_q = (q > 0.0 && q <= 1.0) ? q : 0.99;
+  }
   // Keep freed pool memory for reuse across runs (SpGEMM.cuh:219-223).
   int dev = 0;
   hipMemPool_t pool;
@@ -1076,6 +1198,72 @@ extern "C" int oc_spgemm_set_workflow(oc_spgemm *h, int workflow) {
   OC_REQUIRE(h != nullptr, "null handle");
   OC_REQUIRE(workflow >= -1 && workflow <= 1, "workflow must be -1, 0 or 1");
   h->force_workflow = workflow;
+  return OC_OK;
+}
+
+extern "C" int oc_spgemm_plan(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
+                              void *stream) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  h->drop_plan();
+  h->keep_plan = true;
+  int rc = spgemm_impl(h, dA, dB, dC, (hipStream_t)stream);
+  h->keep_plan = false;
+  if (rc == OC_OK && !h->plan.valid) {
+    h->drop_plan();
+    set_error("oc_spgemm_plan: the product needed an overflow recount; no plan was kept");
+    return OC_ERR_INVALID;
+  }
+  return rc;
+}
+
+extern "C" int oc_spgemm_replay(oc_spgemm *h, void *stream) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  hipStream_t s = (hipStream_t)stream;
+  OC_TRY(replay_impl(h, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return OC_OK;
+}
+
+extern "C" int oc_spgemm_capture(oc_spgemm *h) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  OC_REQUIRE(h->plan.valid, "no numeric plan: run oc_spgemm_plan first");
+  oc_spgemm::Plan &p = h->plan;
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  p.exec = nullptr;
+  p.graph = nullptr;
+  if (!h->own_stream) HIP_TRY(hipStreamCreate(&h->own_stream));
+  HIP_TRY(hipStreamBeginCapture(h->own_stream, hipStreamCaptureModeRelaxed));
+  const int rc = replay_impl(h, h->own_stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->own_stream, &g);
+  if (rc != OC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_TRY(e);
+  p.graph = g;
+  HIP_TRY(hipGraphInstantiate(&p.exec, g, nullptr, nullptr, 0));
+  return OC_OK;
+}
+
+extern "C" int oc_spgemm_graph_launch(oc_spgemm *h, void *stream) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  OC_REQUIRE(h->plan.valid && h->plan.exec, "no captured graph: run oc_spgemm_capture first");
+  if (stream) {
+    HIP_TRY(hipGraphLaunch(h->plan.exec, (hipStream_t)stream));
+  } else {
+    HIP_TRY(hipGraphLaunch(h->plan.exec, h->own_stream));
+    HIP_TRY(hipStreamSynchronize(h->own_stream));
+  }
+  return OC_OK;
+}
+
+extern "C" int oc_spgemm_set_model(oc_spgemm *h, const oc_mlp *m, double quantile) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  OC_REQUIRE(quantile > 0.0 && quantile <= 1.0, "quantile must be in (0, 1]");
+  h->model = m ? m : h->owned_model;
+  h->model_q = quantile;
   return OC_OK;
 }
 

@@ -68,7 +68,7 @@ class _Config(C.Structure):
         ("use_learned_binning", C.c_int32), ("learned_model_file", C.c_char * 512),
         ("use_tiled_numeric", C.c_int32), ("tile_log2_cols", C.c_int32),
         ("tile_block", C.c_int32), ("tile_products_cap", C.c_int32),
-        ("rank_layout", C.c_int32),
+        ("rank_layout", C.c_int32), ("learned_quantile", C.c_double),
     ]
 
 
@@ -139,6 +139,36 @@ def _load():
         "oc_memcpy_dtod": (I, [P, P, C.c_size_t]),
         "oc_memset_device": (I, [P, I, C.c_size_t]),
         "oc_device_synchronize": (I, []),
+        "oc_mlp_create": (P, [U64]),
+        "oc_mlp_destroy": (None, [P]),
+        "oc_mlp_get_params": (I, [P, P]),
+        "oc_mlp_set_params": (I, [P, P]),
+        "oc_mlp_save": (I, [P, C.c_char_p]),
+        "oc_mlp_load": (I, [P, C.c_char_p]),
+        "oc_mlp_class_thresholds": (I, [P]),
+        "oc_mlp_features": (I, [P, P, P, P]),
+        "oc_mlp_labels": (I, [P, U64, P, P]),
+        "oc_mlp_forward": (I, [P, P, U64, P, P]),
+        "oc_mlp_backward": (I, [P, P, P, U64, P]),
+        "oc_mlp_get_grads": (I, [P, P, P]),
+        "oc_mlp_step": (I, [P, C.c_float, P]),
+        "oc_mlp_train": (I, [P, P, P, U64, U32, U32, C.c_float, P, P]),
+        "oc_mlp_estimates": (I, [P, P, P, D, P, P]),
+        "oc_spgemm_set_model": (I, [P, P, D]),
+        "oc_spgemm_plan": (I, [P, P, P, P, P]),
+        "oc_spgemm_replay": (I, [P, P]),
+        "oc_spgemm_capture": (I, [P]),
+        "oc_spgemm_graph_launch": (I, [P, P]),
+        "oc_batch_create": (P, [P]),
+        "oc_batch_destroy": (None, [P]),
+        "oc_batch_set_inputs": (I, [P, P, P, U32, P]),
+        "oc_batch_run": (I, [P, P]),
+        "oc_batch_plan": (I, [P, P]),
+        "oc_batch_capture": (I, [P]),
+        "oc_batch_launch": (I, [P, P]),
+        "oc_batch_result": (I, [P, P, P, P]),
+        "oc_batch_extract": (I, [P, U32, P, P]),
+        "oc_batch_handle": (P, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -417,6 +447,27 @@ class SpGEMM:
     def set_workflow(self, w: int):
         _check(lib.oc_spgemm_set_workflow(self.h, w))
 
+    def plan(self, dA: DeviceCSR, dB: DeviceCSR, stream=None) -> DeviceCSR:
+        """Precise run that keeps the numeric plan for value-only replays."""
+        out = DeviceCSR()
+        _check(lib.oc_spgemm_plan(self.h, C.byref(dA.m), C.byref(dB.m), C.byref(out.m), stream))
+        self._plan_refs = (dA, dB, out)
+        return out
+
+    def replay(self, stream=None):
+        _check(lib.oc_spgemm_replay(self.h, stream))
+
+    def capture(self):
+        _check(lib.oc_spgemm_capture(self.h))
+
+    def graph_launch(self, stream=None):
+        _check(lib.oc_spgemm_graph_launch(self.h, stream))
+
+    def set_model(self, model: "Mlp | None", quantile: float = 0.99):
+        """Learned capacities in place of HyperLogLog (None: back to HLL)."""
+        self._model = model
+        _check(lib.oc_spgemm_set_model(self.h, model.h if model else None, quantile))
+
     def set_estimates(self, buf: "DeviceBuffer | None"):
         self._est = buf
         _check(lib.oc_spgemm_set_estimates(self.h, buf.ptr if buf else None))
@@ -435,6 +486,152 @@ class SpGEMM:
         try:
             if self.h:
                 lib.oc_spgemm_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------- batched
+class Batch:
+    """Independent products C_i = A_i * B_i as one block-diagonal product
+    (include/ocean_capi.h, oc_batch_*)."""
+
+    def __init__(self, config: Config | None = None):
+        self.config = config or Config()
+        self.h = lib.oc_batch_create(C.byref(self.config._c))
+        if not self.h:
+            raise OceanError(OC_ERR_NOMEM, lib.oc_last_error().decode())
+        self.count = 0
+
+    def set_inputs(self, As: list, Bs: list | None = None, stream=None):
+        arr = (_Csr * len(As))(*[a.m for a in As])
+        brr = (_Csr * len(Bs))(*[b.m for b in Bs]) if Bs is not None else None
+        self._inputs = (As, Bs)
+        _check(lib.oc_batch_set_inputs(self.h, arr, brr, len(As), stream))
+        self.count = len(As)
+
+    def run(self, stream=None):
+        _check(lib.oc_batch_run(self.h, stream))
+
+    def plan(self, stream=None):
+        _check(lib.oc_batch_plan(self.h, stream))
+
+    def capture(self):
+        _check(lib.oc_batch_capture(self.h))
+
+    def launch(self, stream=None):
+        _check(lib.oc_batch_launch(self.h, stream))
+
+    def result(self):
+        """(view of C, row_base[count+1], col_base[count+1])."""
+        v = DeviceCSR()
+        rb = np.zeros(self.count + 1, np.uint64)
+        cb = np.zeros(self.count + 1, np.uint64)
+        _check(lib.oc_batch_result(self.h, C.byref(v.m), rb.ctypes.data, cb.ctypes.data))
+        return v, rb, cb
+
+    def extract(self, i: int) -> DeviceCSR:
+        out = DeviceCSR()
+        _check(lib.oc_batch_extract(self.h, i, C.byref(out.m), None))
+        return out
+
+    def get_stats(self) -> dict:
+        hh = lib.oc_batch_handle(self.h)
+        need = C.c_size_t(0)
+        _check(lib.oc_spgemm_stats_json(hh, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        _check(lib.oc_spgemm_stats_json(hh, buf, need.value, C.byref(need)))
+        return json.loads(buf.value.decode())
+
+    def timer_ms(self, name="total") -> float:
+        return lib.oc_spgemm_timer_ms(lib.oc_batch_handle(self.h), name.encode())
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.oc_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------- learned predictor
+MLP_FEATURES, MLP_CLASSES, MLP_NPARAM = 16, 8, 5768
+
+
+class Mlp:
+    """Row-size predictor (include/ocean_capi.h, oc_mlp_*)."""
+
+    def __init__(self, seed: int = 0):
+        self.h = lib.oc_mlp_create(seed)
+        if not self.h:
+            raise OceanError(OC_ERR_NOMEM, lib.oc_last_error().decode())
+
+    def params(self) -> np.ndarray:
+        out = np.empty(MLP_NPARAM, np.float32)
+        _check(lib.oc_mlp_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (MLP_NPARAM,)
+        _check(lib.oc_mlp_set_params(self.h, p.ctypes.data))
+
+    def save(self, path: str):
+        _check(lib.oc_mlp_save(self.h, path.encode()))
+
+    def load(self, path: str):
+        _check(lib.oc_mlp_load(self.h, path.encode()))
+
+    @staticmethod
+    def thresholds() -> np.ndarray:
+        t = np.zeros(MLP_CLASSES, np.uint32)
+        _check(lib.oc_mlp_class_thresholds(t.ctypes.data))
+        return t
+
+    @staticmethod
+    def features(dA: "DeviceCSR", dB: "DeviceCSR") -> DeviceBuffer:
+        buf = DeviceBuffer(4 * MLP_FEATURES * max(dA.rows, 1))
+        _check(lib.oc_mlp_features(C.byref(dA.m), C.byref(dB.m), buf.ptr, None))
+        return buf
+
+    @staticmethod
+    def labels(nnz: DeviceBuffer, n: int) -> DeviceBuffer:
+        buf = DeviceBuffer(max(n, 1))
+        _check(lib.oc_mlp_labels(nnz.ptr, n, buf.ptr, None))
+        return buf
+
+    def forward(self, feat: DeviceBuffer, n: int) -> np.ndarray:
+        out = DeviceBuffer(4 * MLP_CLASSES * max(n, 1))
+        _check(lib.oc_mlp_forward(self.h, feat.ptr, n, out.ptr, None))
+        return out.to_array(np.float32, n * MLP_CLASSES).reshape(n, MLP_CLASSES)
+
+    def backward(self, feat: DeviceBuffer, labels: DeviceBuffer, n: int):
+        _check(lib.oc_mlp_backward(self.h, feat.ptr, labels.ptr, n, None))
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = C.c_float(0)
+        _check(lib.oc_mlp_get_grads(self.h, g.ctypes.data, C.byref(loss)))
+        return g, float(loss.value)
+
+    def step(self, lr: float):
+        _check(lib.oc_mlp_step(self.h, lr, None))
+
+    def train(self, feat: DeviceBuffer, labels: DeviceBuffer, n: int, batch: int,
+              epochs: int, lr: float) -> np.ndarray:
+        losses = np.zeros(max(epochs, 1), np.float32)
+        _check(lib.oc_mlp_train(self.h, feat.ptr, labels.ptr, n, batch, epochs, lr,
+                                losses.ctypes.data, None))
+        return losses[:epochs]
+
+    def estimates(self, dA: "DeviceCSR", dB: "DeviceCSR", quantile: float = 0.99) -> DeviceBuffer:
+        buf = DeviceBuffer(4 * max(dA.rows, 1))
+        _check(lib.oc_mlp_estimates(self.h, C.byref(dA.m), C.byref(dB.m), quantile, buf.ptr, None))
+        return buf
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.oc_mlp_destroy(self.h)
                 self.h = None
         except Exception:
             pass
