@@ -45,6 +45,23 @@ def workload(name: str, ra):
     raise SystemExit(f"unknown workload {name}")
 
 
+def measured_traffic(workload: str, family: str):
+    """HBM bytes per launch of the dominant kernel from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised under profiles/ (a line
+    `traffic_bytes_per_launch <workload> <family> <bytes>`), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for name in sorted(os.listdir(d)):
+        if not name.endswith(".md"):
+            continue
+        for line in open(os.path.join(d, name)):
+            f = line.strip().strip("`").split()
+            if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == workload and f[2] == family:
+                return int(f[3])
+    return None
+
+
 def cpu_baseline(A, B, budget_s: float, nthreads: int):
     """Oracle Gustavson on row panels spread over the matrix."""
     from oracle import oracle as ora
@@ -86,6 +103,8 @@ def main():
     ap.add_argument("--workload", default="rmat20", choices=["rmat20", "laplacian", "rect"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -94,6 +113,19 @@ def main():
 
     import research_amd as ra
     from research_amd import ocean
+
+    if args.calibrate_copy:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this run give
+        # the factors that turn those counters into bytes on this device
+        n = 1 << 30
+        src, dst = ra.DeviceBuffer(n), ra.DeviceBuffer(n)
+        src.zero()
+        for _ in range(3):
+            ocean._check(ocean.lib.oc_memcpy_dtod(dst.ptr, src.ptr, n))
+        ocean.lib.oc_device_synchronize()
+        print(json.dumps({"calibration": "device-to-device copy", "bytes_read": n,
+                          "bytes_written": n, "copies": 3}), flush=True)
+        return
 
     t0 = time.perf_counter()
     A, B, desc, dtype = workload(args.workload, ra)
@@ -139,7 +171,8 @@ def main():
         avg_ms = fam_ms[dom] / args.steps
         ach = fam_bytes[dom] / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": PEAK_HBM_GBPS,
-                "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBPS, 4),
+                "traffic": measured_traffic(args.workload, dom),
                 "avg_launch_ms": round(avg_ms, 3), "bytes_alg_per_launch": fam_bytes[dom]}
     whole = st.get("roofline", {})
     out = {

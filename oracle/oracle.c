@@ -215,6 +215,10 @@ void ora_analysis(uint64_t a_rows, const uint32_t *a_rowptr,
 /* ------------------------------------------------------------------------ */
 /* Binning                                                                   */
 /* ------------------------------------------------------------------------ */
+/* Expanded sizes saturate at 2^32-1 (the reference's cast of an
+ * out-of-range value is undefined; the GPU binning saturates the same way). */
+static uint32_t sat_u32(double x) { return x >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)x; }
+
 void ora_bin_symbolic(uint64_t rows, const uint32_t *products,
                       const uint32_t *b_left, const uint32_t *b_right,
                       const uint32_t *hash_sizes, const uint32_t *dense_sizes,
@@ -232,7 +236,7 @@ void ora_bin_symbolic(uint64_t rows, const uint32_t *products,
     }
     uint32_t val = products[r];
     if (use_compaction) val = (uint32_t)(val / avg_compaction); /* :150-152 */
-    val = (uint32_t)(val * expand_factor);                      /* :153-155 */
+    val = sat_u32((double)val * expand_factor);                 /* :153-155 */
     int bin = 6;
     for (int b = 0; b < 6; ++b) {
       if (val <= hash_sizes[b]) {
@@ -278,7 +282,7 @@ void ora_bin_numeric(uint64_t rows, uint32_t *input, const uint32_t *b_left,
       }
     }
     uint32_t val = input[r];
-    if (expand) val = (uint32_t)(val * expand_factor);
+    if (expand) val = sat_u32((double)val * expand_factor);
     int bin = numeric_bin_one(val, dense_bin, 6, hash_sizes);
     if (use_largest) {
       int cand = numeric_bin_one(val, dense_bin, 7, hash_sizes);

@@ -301,7 +301,7 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
   uint32_t mine = 0;
   bool ovf = false;
   for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
-    if (track && lds_load(&hdr[0]) >= limit) {
+    if (track && lds_load(&hdr[0]) > limit) {  // past the load limit
       ovf = true;
       break;
     }
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
   if (EST) limit = min(L.nmax - 1, o.cap[r]);
   bool ovf = false;
   for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
-    if (EST && lds_load(&hdr[0]) >= limit) {
+    if (EST && lds_load(&hdr[0]) > limit) {  // more keys than the capacity
       ovf = true;
       break;
     }

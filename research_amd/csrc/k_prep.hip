@@ -548,7 +548,7 @@ __device__ __forceinline__ int sym_bin(uint32_t products, uint32_t lo,
     }
   uint32_t val = products;
   if (use_compaction) val = (uint32_t)(val / comp);
-  val = (uint32_t)(val * expand);
+  val = (uint32_t)fmin((double)val * expand, 4294967295.0);  // saturating
   int bin = 6;
   for (int b = 0; b < 6; ++b)
     if (val <= bt.hash_sym[b]) {
@@ -598,7 +598,7 @@ __device__ __forceinline__ void num_prepare(const BinArgs &ba, const BinTables &
       break;
     }
   val = in;
-  if (ba.expand) val = (uint32_t)(val * ba.expand_factor);
+  if (ba.expand) val = (uint32_t)fmin((double)val * ba.expand_factor, 4294967295.0);
 }
 
 // Hybrid candidates (numeric bin 6 when allowed) for the deterministic cap.

@@ -696,6 +696,13 @@ int Run::run_est() {
   bp.modify_dense_est = 1;
   bp.use_largest = d.hash_largest;
   bp.hybrid_max = (uint32_t)(m / 10);
+  // HyperLogLog estimates carry their expansion already; capacities given by
+  // the caller or the predictor are counts, binned with the numeric
+  // expansion like exact counts (a hash table of S slots takes 2S/3 keys)
+  if (h->est_override || h->learned_run) {
+    bp.expand = 1;
+    bp.expand_factor = c.numeric_expansion_coe;
+  }
   uint8_t *bor = nullptr;
   uint32_t *brows = nullptr, *boff = nullptr;
   uint32_t hoff[kBinSlots + 1];

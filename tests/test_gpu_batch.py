@@ -38,13 +38,12 @@ def test_plan_replay_graph(gpu, kind):
         eager = gpu.SpGEMM(gpu.Config()).run(dA, dA).download()
         assert np.array_equal(eager.col_ids, got.col_ids)
         log(f"{kind} replay {it} ({'eager' if it == 0 else 'graph'}): err {worst:.2e}")
-    # a graph replay with unchanged values reproduces the previous result
-    before = dC.download().values.copy()
+    # graph replays with unchanged values keep the result (sums may be
+    # reordered by the LDS FP64 atomics, so compare against the oracle)
     sp.graph_launch()
     sp.graph_launch()
-    after = dC.download().values
-    err = np.abs(after - before) / np.maximum(np.abs(before), 1e-300)
-    assert float(err.max()) < 1e-12
+    worst, _ = check_product(A, A, dC.download(), 1e-12, f"{kind} repeated graph")
+    log(f"{kind} repeated graph launches: err {worst:.2e}")
 
 
 def _mixed_batch(gpu, count, vtype, seed):

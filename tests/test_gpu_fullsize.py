@@ -108,12 +108,12 @@ def test_estimation_overflow_forced(gpu, kind):
         est[cand[::5]] = 1
     if kind == "long_rows":
         bins0, _ = ora.bin_numeric(est, ref["left"], ref["right"], t["hash_num"], t["dense_num"],
-                                   0, 0.0, modify_dense_est=1)
+                                   1, 1.5, modify_dense_est=1)
         longr = np.nonzero(bins0 == 13)[0]
         assert len(longr) >= 3, "no long rows in this matrix"
         est[longr[:3]] = exact[longr[:3]] // 2
     bins, est_mod = ora.bin_numeric(est, ref["left"], ref["right"], t["hash_num"], t["dense_num"],
-                                    0, 0.0, modify_dense_est=1)
+                                    1, 1.5, modify_dense_est=1)
     want_ovf = int(np.count_nonzero(exact > est_mod))
     dA = A.upload()
     sp = gpu.SpGEMM(gpu.Config(use_ultrasparse_workflow=0))
