@@ -45,6 +45,108 @@ def workload(name: str, ra):
     raise SystemExit(f"unknown workload {name}")
 
 
+def batched_products(ra, count: int, seed: int = 4):
+    """The batched configuration: `count` independent products C_i = A_i*A_i,
+    n log-uniform in [500, 50000], type cycling Erdos-Renyi (degree 2..32),
+    banded (bandwidth 3..65) and R-MAT (degree 8, n rounded to 2^k), FP32
+    values U[-1,1)."""
+    rng = np.random.default_rng(seed)
+    mats = []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(500), np.log(50000))))
+        s = seed * 1_000_000 + i
+        if i % 3 == 0:
+            M = ra.CSR.erdos_renyi(n, float(rng.integers(2, 33)), vtype=1, seed=s)
+        elif i % 3 == 1:
+            M = ra.CSR.banded(n, int(rng.integers(1, 33)) * 2 + 1, vtype=1, seed=s)
+        else:
+            lg = max(9, int(round(np.log2(n))))
+            M = ra.CSR.rmat(lg, 8 << lg, vtype=1, seed=s)
+        mats.append(M)
+    return mats
+
+
+def product_count(M) -> int:
+    rl = np.diff(M.row_offsets.astype(np.int64))
+    return int(rl[M.col_ids.astype(np.int64)].sum())
+
+
+def run_batched(ra, ocean, count: int, steps: int, model_path: str | None):
+    """Plain path (one oc_spgemm_run per product, HyperLogLog-binned) against
+    one block-diagonal batched run (HyperLogLog and learned capacities) and
+    the captured graph that recomputes every product's values."""
+    t0 = time.perf_counter()
+    mats = batched_products(ra, count)
+    products = sum(product_count(M) for M in mats)
+    dev = [M.upload() for M in mats]
+    gen_s = time.perf_counter() - t0
+    ocean.lib.oc_device_synchronize()
+    # plain: one product at a time
+    sp = ra.SpGEMM(ra.Config())
+    lat = []
+    t0 = time.perf_counter()
+    for d in dev:
+        t1 = time.perf_counter()
+        C = sp.run(d, d)
+        C.free()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    plain_s = time.perf_counter() - t0
+    # batched run (whole path once over the block-diagonal operands)
+    b = ra.Batch(ra.Config())
+    b.set_inputs(dev)
+    b.run()
+    ocean.lib.oc_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+    ocean.lib.oc_device_synchronize()
+    batched_s = (time.perf_counter() - t0) / steps
+    ref_view, _, _ = b.result()
+    ref_hash = ref_view.row_set_hash()
+    out = {"products_count": count, "products": products, "generation_s": round(gen_s, 1),
+           "plain_gflops": round(2 * products / plain_s / 1e9, 3),
+           "plain_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+           "plain_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+           "batched_gflops": round(2 * products / batched_s / 1e9, 3),
+           "batched_latency_ms": round(batched_s * 1e3, 3),
+           "batched_workflow": b.get_stats()["decision"]["workflow"]}
+    # learned capacities on the batched run
+    if model_path and os.path.exists(model_path):
+        m = ra.Mlp(0)
+        m.load(model_path)
+        ocean._check(ocean.lib.oc_spgemm_set_model(ocean.lib.oc_batch_handle(b.h), m.h, 0.99))
+        b.run()
+        ocean.lib.oc_device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run()
+        ocean.lib.oc_device_synchronize()
+        learned_s = (time.perf_counter() - t0) / steps
+        view, _, _ = b.result()
+        st = b.get_stats()
+        out.update({"learned_batched_gflops": round(2 * products / learned_s / 1e9, 3),
+                    "learned_batched_latency_ms": round(learned_s * 1e3, 3),
+                    "learned_overflow_rows": st.get("estimation", {}).get("overflow_rows"),
+                    "learned_same_structure": bool(np.array_equal(view.row_set_hash(), ref_hash)),
+                    "learned_batched_vs_plain": round(plain_s / learned_s, 2)})
+        ocean._check(ocean.lib.oc_spgemm_set_model(ocean.lib.oc_batch_handle(b.h), None, 0.99))
+    # value-only graph replays of the planned batch
+    b.plan()
+    b.capture()
+    b.launch()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.launch()
+    graph_s = (time.perf_counter() - t0) / steps
+    view, _, _ = b.result()
+    out.update({"graph_gflops": round(2 * products / graph_s / 1e9, 3),
+                "graph_latency_ms": round(graph_s * 1e3, 3),
+                "graph_same_structure": bool(np.array_equal(view.row_set_hash(), ref_hash)),
+                "batched_vs_plain": round(plain_s / batched_s, 2),
+                "graph_vs_plain": round(plain_s / graph_s, 2)})
+    return out
+
+
 def measured_traffic(workload: str, family: str):
     """HBM bytes per launch of the dominant kernel from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs summarised under profiles/ (a line
@@ -100,7 +202,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "laplacian", "rect"])
+    ap.add_argument("--workload", default="rmat20",
+                    choices=["rmat20", "laplacian", "rect", "batched"])
+    ap.add_argument("--batch-count", type=int, default=None,
+                    help="products in the batched comparison (4096 for --workload batched, "
+                         "512 beside the default workload, 0 to skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -125,6 +231,22 @@ def main():
         ocean.lib.oc_device_synchronize()
         print(json.dumps({"calibration": "device-to-device copy", "bytes_read": n,
                           "bytes_written": n, "copies": 3}), flush=True)
+        return
+
+    model_path = os.path.join(ROOT, "research_amd", "models", "rowsize_mlp.bin")
+    if args.workload == "batched":
+        n = args.batch_count or 4096
+        res = run_batched(ra, ocean, n, args.steps, model_path)
+        best = max(res["batched_gflops"], res.get("learned_batched_gflops", 0.0))
+        out = {"metric": METRIC, "value": best, "unit": "GFLOP/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": 1, "ms_per_step": res["batched_latency_ms"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+               "data": "synthetic",
+               "config": {"workload": f"{n} independent products C_i = A_i*A_i (ER / banded / "
+                                      f"R-MAT, n log-uniform 500..50000, FP32, seed 4)",
+                          "parallelism": "single GPU"},
+               "batched": res}
+        print(json.dumps(out), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -202,6 +324,12 @@ def main():
     if not args.no_cpu_baseline:
         nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
         out["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds, nth)
+    nb = 512 if args.batch_count is None else args.batch_count
+    if nb > 0:
+        dA.free()
+        if dB is not dA:
+            dB.free()
+        out["batched"] = run_batched(ra, ocean, nb, max(1, min(args.steps, 3)), model_path)
     print(json.dumps(out), flush=True)
 
 

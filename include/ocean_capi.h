@@ -124,8 +124,12 @@ typedef struct oc_config {
   int32_t tile_log2_cols;  /* tile width 2^k columns (10..16), default 14 */
   int32_t tile_block;      /* workgroup size of the tiled kernel: 256/512/1024 */
   int32_t tile_products_cap; /* rank tiles: products per tile target (0 = none) */
-  int32_t rank_layout;     /* rank tiles: products per lane 0 strided, 1 contiguous */
   double learned_quantile; /* predictor: cumulative class probability to cover */
+  /* Long rows of the precise workflow swept in column bands of
+   * 2^band_log2_cols (10..15; 0 = off, the default) across all rows at once, for rows
+   * with at least band_min_products products and a wider span. */
+  int32_t band_log2_cols;
+  int32_t band_min_products;
 } oc_config;
 
 /* Replaces: Config's default member initialisers (include/Utils.h:22-107). */

@@ -94,6 +94,25 @@ __device__ __forceinline__ T block_excl_sum(T v, T *scratch, T *total) {
   return r;
 }
 
+// Block-wide exclusive max (identity 0) for blockDim.x <= 1024; `scratch`
+// holds 16 entries.  Returns the max over the threads before this one.
+template <typename T>
+__device__ __forceinline__ T block_excl_max(T v, T *scratch) {
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  T incl = v;
+  for (int d = 1; d < kWave; d <<= 1) {
+    const T o = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl = o > incl ? o : incl;
+  }
+  T excl = __shfl_up(incl, 1, kWave);
+  if (lane == 0) excl = T(0);
+  if (lane == kWave - 1) scratch[wid] = incl;
+  __syncthreads();
+  for (int w = 0; w < wid; ++w) excl = scratch[w] > excl ? scratch[w] : excl;
+  __syncthreads();
+  return excl;
+}
+
 // ------------------------------------------------------------- hashing
 // Linear-probing slot of a column: Fibonacci hash, then Lemire range
 // reduction onto a table of any size (no integer division).

@@ -62,8 +62,9 @@ extern "C" void oc_config_default(oc_config *c) {
   c->tile_log2_cols = 14;
   c->tile_block = 1024;
   c->tile_products_cap = 0;
-  c->rank_layout = 0;
   c->learned_quantile = 0.99;
+  c->band_log2_cols = 0;
+  c->band_min_products = 16384;
 }
 
 namespace {
@@ -278,8 +279,9 @@ extern "C" int oc_config_from_json(oc_config *c, const char *text) {
     I("tile_log2_cols", tile_log2_cols)
     I("tile_block", tile_block)
     I("tile_products_cap", tile_products_cap)
-    I("rank_layout", rank_layout)
     D("learned_quantile", learned_quantile)
+    I("band_log2_cols", band_log2_cols)
+    I("band_min_products", band_min_products)
 #undef B
 #undef I
 #undef D

@@ -874,67 +874,21 @@ __global__ __launch_bounds__(kTileBlock) void k_num_tiles(
 // tile is written with coalesced stores.  Tiles with more outputs than
 // slots are finished in rank ranges.  Segment bounds come from the
 // tile-start table (granularity 2^logT columns).
-constexpr int kRankBlock = 1024;
-constexpr uint32_t kRankA = 1024;
-constexpr uint32_t kRankMaxLog = 17;
-constexpr uint32_t kRankWords = (1u << kRankMaxLog) / 32;
-
-static size_t rank_lds_fixed() {
-  return 256 + (size_t)kRankA * (4 + 4 + 4 + 8) + (size_t)kRankWords * 8;
-}
-
-__device__ __forceinline__ uint32_t owner_of(const uint32_t *spre, uint32_t nc, uint32_t p) {
-  uint32_t lo = 0, hb = nc - 1;
-  while (lo < hb) {
-    const uint32_t mid = (lo + hb) >> 1;
-    if (spre[mid] > p) hb = mid;
-    else lo = mid + 1;
+// Shape of a rank-tile workgroup of BS threads: A entries staged per chunk,
+// products per owner map, widest tile (2^MAXLOG columns) and its bitmap
+// words.  Smaller workgroups take a proportional share of the CU's LDS, so
+// several rows are in flight per CU and one row's LDS phases overlap
+// another's gathers.
+template <int BS>
+struct RankShape {
+  static constexpr uint32_t A = BS;
+  static constexpr uint32_t ROUND = 8 * BS;
+  static constexpr uint32_t MAXLOG = BS >= 1024 ? 17 : (BS >= 512 ? 16 : 15);
+  static constexpr uint32_t WORDS = (1u << MAXLOG) / 32;
+  static size_t lds_fixed() {
+    return 256 + (size_t)A * (4 + 4 + 8) + (size_t)ROUND * 2 + (size_t)WORDS * 8;
   }
-  return lo;
-}
-
-// Products pb .. pb+7 (< P) of one lane in a chunk's flattened product
-// space: B entry q[u] and owning A entry w[u].  One binary search for the
-// first, then the owner only advances at segment ends.
-__device__ __forceinline__ void locate8(const uint32_t *spre, const uint32_t *sst,
-                                        uint32_t nc, uint32_t pb, uint32_t P,
-                                        uint32_t (&q)[8], uint32_t (&w)[8]) {
-  uint32_t o = pb < P ? owner_of(spre, nc, pb) : 0u;
-  uint32_t e = spre[o];
-  uint32_t sb = o ? spre[o - 1] : 0u;
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const uint32_t p = pb + u;
-    q[u] = kEmpty;
-    w[u] = 0;
-    if (p < P) {
-      while (p >= e) {
-        ++o;
-        sb = e;
-        e = spre[o];
-      }
-      q[u] = sst[o] + (p - sb);
-      w[u] = o;
-    }
-  }
-}
-
-// Lane-coalesced variant: products pb + u*1024 (one binary search each).
-__device__ __forceinline__ void locate8_strided(const uint32_t *spre, const uint32_t *sst,
-                                                uint32_t nc, uint32_t pb, uint32_t P,
-                                                uint32_t (&q)[8], uint32_t (&w)[8]) {
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const uint32_t p = pb + u * 1024u;
-    q[u] = kEmpty;
-    w[u] = 0;
-    if (p < P) {
-      const uint32_t o = owner_of(spre, nc, p);
-      q[u] = sst[o] + (p - (o ? spre[o - 1] : 0u));
-      w[u] = o;
-    }
-  }
-}
+};
 
 // OR the column bits of up to 8 products into the tile bitmap, combining
 // products that fall into the same 32-column word.
@@ -959,24 +913,31 @@ __device__ __forceinline__ uint32_t rank_of(uint32_t x, const uint32_t *wpre, co
   return wpre[wd] + __popc(bits[wd] & ((1u << (x & 31)) - 1u));
 }
 
-template <typename T, bool CONTIG>
-__global__ __launch_bounds__(kRankBlock) void k_num_rank(
+// One workgroup per long row (rows taken from an atomic counter).  A
+// chunk's products are flattened (A entry i owns positions [ex_i,
+// ex_i + len_i)) and enumerated in rounds of 8192: an LDS owner map, built
+// by marking each segment's first position and a workgroup max-scan, gives
+// every position its A entry with one LDS read, and lane-strided positions
+// keep the B gathers coalesced.
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void k_num_rank(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
     uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1, uint32_t cap, uint32_t pcap,
     KOut<T> o) {
   extern __shared__ __align__(16) uint8_t smem[];
   uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
   uint32_t *sk = reinterpret_cast<uint32_t *>(smem + 256);
-  uint32_t *sst = sk + kRankA;
-  uint32_t *spre = sst + kRankA;
-  double *sav = reinterpret_cast<double *>(spre + kRankA);
-  uint32_t *bits = reinterpret_cast<uint32_t *>(sav + kRankA);
-  uint32_t *wpre = bits + kRankWords;
-  double *acc = reinterpret_cast<double *>(wpre + kRankWords);
+  uint32_t *sbase = sk + RankShape<BS>::A;  // B position of product p is sbase[owner] + p
+  double *sav = reinterpret_cast<double *>(sbase + RankShape<BS>::A);
+  uint16_t *own = reinterpret_cast<uint16_t *>(sav + RankShape<BS>::A);
+  uint32_t *bits = reinterpret_cast<uint32_t *>(own + RankShape<BS>::ROUND);
+  uint32_t *wpre = bits + RankShape<BS>::WORDS;
+  double *acc = reinterpret_cast<double *>(wpre + RankShape<BS>::WORDS);
   uint32_t *cmap = reinterpret_cast<uint32_t *>(acc + cap);
+  typedef unsigned short us8 __attribute__((ext_vector_type(8)));
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < kRankWords; i += kRankBlock) bits[i] = 0;
-  for (uint32_t i = tid; i < cap; i += kRankBlock) acc[i] = 0.0;
+  for (uint32_t i = tid; i < RankShape<BS>::WORDS; i += BS) bits[i] = 0;
+  for (uint32_t i = tid; i < cap; i += BS) acc[i] = 0.0;
   __syncthreads();
   for (;;) {
     if (tid == 0) hdr[0] = atomicAdd(counter, 1u);
@@ -997,12 +958,12 @@ __global__ __launch_bounds__(kRankBlock) void k_num_rank(
     if (o.nnz_hint) est = o.nnz_hint[r];
     const uint64_t prod = a.products[r];
     uint32_t lw = logT;
-    while (lw < kRankMaxLog && (uint64_t)(span >> lw) >= 1 &&
+    while (lw < RankShape<BS>::MAXLOG && (uint64_t)(span >> lw) >= 1 &&
            est * (1ull << (lw + 1)) <= (uint64_t)cap * 3 / 4 * span &&
            (pcap == 0 || prod * (1ull << (lw + 1)) <= (uint64_t)pcap * span))
       ++lw;
     const uint32_t step = lw - logT;
-    const bool one_chunk = nA <= kRankA;
+    const bool one_chunk = nA <= RankShape<BS>::A;
     const uint64_t off = o.off[r];
     // estimated capacity (estimation workflow): a row whose outputs exceed
     // it stops before writing past its region and joins the overflow list
@@ -1011,8 +972,9 @@ __global__ __launch_bounds__(kRankBlock) void k_num_rank(
     uint64_t total = 0;
     const uint32_t t_begin = left >> lw, t_end = hi >> lw;
     const uint32_t nwords = 1u << (lw - 5);
-    const uint32_t wpt = (nwords + kRankBlock - 1) / kRankBlock;
+    const uint32_t wpt = (nwords + BS - 1) / BS;
     const uint32_t b0 = min(nwords, tid * wpt), b1 = min(nwords, b0 + wpt);
+    uint32_t my_ex = 0, my_len = 0;  // this thread's A entry segment
 
     // stage chunk c0.. of the A row and the segment bounds of tile t
     auto stage = [&](uint32_t c0, uint32_t nc) {
@@ -1030,36 +992,68 @@ __global__ __launch_bounds__(kRankBlock) void k_num_rank(
         len = ts[base + min(T1 - 1, (t + 1) << step)] - st;
       }
       const uint32_t ex = block_excl_sum(len, hdr + 8, &P);
-      if (tid < nc) {
-        sst[tid] = st;
-        spre[tid] = ex + len;
-      }
+      my_ex = ex;
+      my_len = len;
+      if (tid < nc) sbase[tid] = st - ex;
       __syncthreads();
+    };
+    // owner map of positions [p0, p0 + n)
+    auto build_map = [&](uint32_t p0, uint32_t n) {
+      __syncthreads();  // the previous map is no longer read
+      if (8 * tid < n) *reinterpret_cast<us8 *>(own + 8 * tid) = us8{0, 0, 0, 0, 0, 0, 0, 0};
+      __syncthreads();
+      if (my_len > 0 && my_ex < p0 + n && my_ex + my_len > p0)
+        own[my_ex > p0 ? my_ex - p0 : 0] = (uint16_t)tid;
+      __syncthreads();
+      us8 v = (8 * tid < n) ? *reinterpret_cast<const us8 *>(own + 8 * tid) : us8{0, 0, 0, 0, 0, 0, 0, 0};
+      uint32_t run = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        run = max(run, (uint32_t)v[j]);
+        v[j] = (unsigned short)run;
+      }
+      const uint32_t pre = block_excl_max(run, hdr + 32);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = (unsigned short)max(pre, (uint32_t)v[j]);
+      if (8 * tid < n) *reinterpret_cast<us8 *>(own + 8 * tid) = v;
+      __syncthreads();
+    };
+    // products p0 + tid + 1024 u of the current map: B entry q[u], A entry w[u]
+    auto locate = [&](uint32_t p0, uint32_t n, uint32_t (&q)[8], uint32_t (&w)[8]) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t i = tid + (uint32_t)u * BS;
+        q[u] = kEmpty;
+        w[u] = 0;
+        if (i < n) {
+          const uint32_t ow = own[i];
+          q[u] = sbase[ow] + p0 + i;
+          w[u] = ow;
+        }
+      }
     };
     if (one_chunk) stage(a0, nA);
     for (uint32_t t = t_begin; t <= t_end; ++t) {
       const uint64_t wbase = (uint64_t)t << lw;
       // ---- pass 1: mark the output columns
-      uint32_t any = 0;
+      uint32_t any = 0, Plast = 0;
       bool keep = false;
-      uint32_t kc[8];
-      double kv[8];
-      for (uint32_t c0 = a0; c0 < a1; c0 += kRankA) {
-        const uint32_t nc = min(kRankA, a1 - c0);
+      uint32_t kc[8];  // kept columns of a single-round tile
+      for (uint32_t c0 = a0; c0 < a1; c0 += RankShape<BS>::A) {
+        const uint32_t nc = min(RankShape<BS>::A, a1 - c0);
         if (!one_chunk) stage(c0, nc);
         uint32_t P;
         bounds(t, nc, P);
         any |= P;
-        keep = one_chunk && P <= 8 * kRankBlock;
-        for (uint32_t p0 = 0; p0 < P; p0 += 8 * kRankBlock) {
+        Plast = P;
+        keep = one_chunk && P <= RankShape<BS>::ROUND;
+        for (uint32_t p0 = 0; p0 < P; p0 += RankShape<BS>::ROUND) {
+          const uint32_t n = min(RankShape<BS>::ROUND, P - p0);
+          build_map(p0, n);
           uint32_t q[8], w[8];
-          if (CONTIG) locate8(spre, sst, nc, p0 + 8 * tid, P, q, w);
-          else locate8_strided(spre, sst, nc, p0 + tid, P, q, w);
+          locate(p0, n, q, w);
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            kc[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
-            if (keep) kv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
-          }
+          for (int u = 0; u < 8; ++u) kc[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
           mark8(kc, wbase, bits);
         }
         if (!one_chunk) __syncthreads();
@@ -1086,27 +1080,34 @@ __global__ __launch_bounds__(kRankBlock) void k_num_rank(
       for (uint32_t r0 = 0; r0 < m; r0 += cap) {
         const uint32_t r1 = min(m, r0 + cap);
         if (keep) {
+          // the single round's owner map is still in LDS: gather the values
+          uint32_t q[8], w[8];
+          locate(0, Plast, q, w);
+          double pv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
 #pragma unroll
           for (int u = 0; u < 8; ++u)
             if (kc[u] != kEmpty) {
               const uint32_t rk = rank_of((uint32_t)(kc[u] - wbase), wpre, bits);
               if (rk >= r0 && rk < r1) {
-                lds_add_f64(&acc[rk - r0], kv[u]);
+                lds_add_f64(&acc[rk - r0], pv[u]);
                 cmap[rk - r0] = kc[u];
               }
             }
         } else {
-          for (uint32_t c0 = a0; c0 < a1; c0 += kRankA) {
-            const uint32_t nc = min(kRankA, a1 - c0);
-            uint32_t P = spre[nc - 1];
+          for (uint32_t c0 = a0; c0 < a1; c0 += RankShape<BS>::A) {
+            const uint32_t nc = min(RankShape<BS>::A, a1 - c0);
+            uint32_t P = Plast;
             if (!one_chunk) {
               stage(c0, nc);
               bounds(t, nc, P);
             }
-            for (uint32_t p0 = 0; p0 < P; p0 += 8 * kRankBlock) {
+            for (uint32_t p0 = 0; p0 < P; p0 += RankShape<BS>::ROUND) {
+              const uint32_t n = min(RankShape<BS>::ROUND, P - p0);
+              build_map(p0, n);
               uint32_t q[8], w[8];
-              if (CONTIG) locate8(spre, sst, nc, p0 + 8 * tid, P, q, w);
-              else locate8_strided(spre, sst, nc, p0 + tid, P, q, w);
+              locate(p0, n, q, w);
               uint32_t col[8];
               double pv[8];
 #pragma unroll
@@ -1129,7 +1130,7 @@ __global__ __launch_bounds__(kRankBlock) void k_num_rank(
         }
         __syncthreads();
         // coalesced write-out of ranks [r0, r1)
-        for (uint32_t i = tid; i < r1 - r0; i += kRankBlock) {
+        for (uint32_t i = tid; i < r1 - r0; i += BS) {
           o.cols[off + total + r0 + i] = cmap[i];
           o.vals[off + total + r0 + i] = (T)acc[i];
           acc[i] = 0.0;
@@ -1148,33 +1149,37 @@ __global__ __launch_bounds__(kRankBlock) void k_num_rank(
 }
 
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                    const uint32_t *ts, uint32_t T1, uint32_t pcap, int contig,
+                    const uint32_t *ts, uint32_t T1, uint32_t pcap, uint32_t bs,
                     const NumOut &o, void *scratch, hipStream_t s) {
   if (nrows == 0) return OC_OK;
   const Tables &t = tables();
-  const size_t fixed = rank_lds_fixed();
-  if (fixed + 64 * 12 > t.lds_max_block) {
-    set_error("rank-tile kernel does not fit the LDS of this device");
-    return OC_ERR_INVALID;
-  }
-  const uint32_t cap = (uint32_t)(((t.lds_max_block - fixed) / 12) & ~63ull);
-  const size_t lds = fixed + (size_t)cap * 12;
   uint32_t *counter = (uint32_t *)scratch;
   OC_TRY(zero_u32(counter, 1, s));
-  const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu));
-#define GO(T, C)                                                                    \
+  if (bs != 256 && bs != 512) bs = 1024;
+  const uint32_t per_cu = 1024 / bs;  // workgroups resident per CU
+  const size_t budget = std::min<size_t>(t.lds_max_block, t.lds_per_cu / per_cu) & ~255ull;
+  const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu) * per_cu);
+#define GO(T, B)                                                                    \
   do {                                                                              \
-    OC_TRY(set_lds(k_num_rank<T, C>, lds));                                         \
-    hipLaunchKernelGGL((k_num_rank<T, C>), dim3(grid), dim3(kRankBlock), lds, s,    \
-                       kargs<T>(a), rows, nrows, counter, logT, ts, T1, cap, pcap,  \
-                       kout<T>(o));                                                 \
+    const size_t fixed = RankShape<B>::lds_fixed();                                 \
+    if (fixed + 64 * 12 > budget) {                                                 \
+      set_error("rank-tile kernel does not fit the LDS of this device");            \
+      return OC_ERR_INVALID;                                                        \
+    }                                                                               \
+    const uint32_t cap = (uint32_t)(((budget - fixed) / 12) & ~63ull);              \
+    const size_t lds = fixed + (size_t)cap * 12;                                    \
+    OC_TRY(set_lds(k_num_rank<T, B>, lds));                                         \
+    hipLaunchKernelGGL((k_num_rank<T, B>), dim3(grid), dim3(B), lds, s, kargs<T>(a), \
+                       rows, nrows, counter, logT, ts, T1, cap, pcap, kout<T>(o));  \
   } while (0)
   if (a.A.vtype == OC_F64) {
-    if (contig) GO(double, true);
-    else GO(double, false);
+    if (bs == 256) GO(double, 256);
+    else if (bs == 512) GO(double, 512);
+    else GO(double, 1024);
   } else {
-    if (contig) GO(float, true);
-    else GO(float, false);
+    if (bs == 256) GO(float, 256);
+    else if (bs == 512) GO(float, 512);
+    else GO(float, 1024);
   }
 #undef GO
   LAUNCH_CHECK();

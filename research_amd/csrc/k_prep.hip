@@ -581,8 +581,13 @@ struct BinArgs {
   uint32_t hybrid_max;
   const uint32_t *filter;
   uint32_t skip_below;
+  const uint32_t *skip_flag;    // rows handled elsewhere (long-row bands)
   const uint64_t *hybrid_rank;  // exclusive rank among hybrid candidates
 };
+
+__device__ __forceinline__ bool skipped(const BinArgs &ba, uint32_t r) {
+  return (ba.filter && ba.filter[r] <= ba.skip_below) || (ba.skip_flag && ba.skip_flag[r]);
+}
 
 // numeric: compute dense bin and the (possibly modified) value
 __device__ __forceinline__ void num_prepare(const BinArgs &ba, const BinTables &bt,
@@ -609,14 +614,13 @@ __global__ void k_bin_hybrid_flags(BinArgs ba, BinTables bt, uint32_t rows,
     int db;
     uint32_t val;
     num_prepare(ba, bt, r, db, val);
-    bool skip = ba.filter && ba.filter[r] <= ba.skip_below;
-    flags[r] = (!skip && num_bin_one(val, db, 7, bt) == 6) ? 1u : 0u;
+    flags[r] = (!skipped(ba, r) && num_bin_one(val, db, 7, bt) == 6) ? 1u : 0u;
   }
 }
 
 __device__ __forceinline__ int classify(const BinArgs &ba, const BinTables &bt,
                                         uint32_t r) {
-  if (ba.filter && ba.filter[r] <= ba.skip_below) return kSkipBin;
+  if (skipped(ba, r)) return kSkipBin;
   if (ba.kind == 0)
     return sym_bin(ba.input[r], ba.left[r], ba.right[r], ba.use_compaction,
                    ba.expand_factor, ba.avg_compaction, bt);
@@ -641,7 +645,7 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_classify(
     const int bin = classify(ba, bt, r);
     if (ba.kind == 1 && ba.modify_dense_est && ba.input_mut) {
       const uint32_t range = ba.right[r] - ba.left[r] + 1u;
-      if (range <= bt.dense_num[0] && !(ba.filter && ba.filter[r] <= ba.skip_below))
+      if (range <= bt.dense_num[0] && !skipped(ba, r))
         ba.input_mut[r] = bt.dense_num[0];
     }
     bin_of_row[r] = (uint8_t)bin;
@@ -744,6 +748,7 @@ int launch_binning(const BinParams &bp, uint64_t rows, uint8_t *bin_of_row,
   ba.hybrid_max = bp.hybrid_max;
   ba.filter = bp.filter;
   ba.skip_below = bp.skip_below;
+  ba.skip_flag = bp.skip_flag;
   if (rows == 0) {
     HIP_TRY(hipMemsetAsync(bin_offsets, 0, 4 * (kBinSlots + 1), s));
     return OC_OK;

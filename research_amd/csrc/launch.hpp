@@ -78,6 +78,7 @@ struct BinParams {
   uint32_t hybrid_max;
   const uint32_t *filter;
   uint32_t skip_below;
+  const uint32_t *skip_flag;  // rows with a non-zero flag go to the skip bin
 };
 size_t bin_temp_bytes(uint64_t rows);
 int launch_binning(const BinParams &bp, uint64_t rows, uint8_t *bin_of_row,
@@ -159,9 +160,8 @@ int launch_num_tiles(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uin
                      uint32_t bs, const uint32_t *ts, uint32_t T1, const NumOut &o,
                      void *scratch, hipStream_t s, const uint32_t *nrows_dev = nullptr);
 // Rank-compressed tiles (tile-start table of granularity 2^logT columns).
-// contig: products laid out contiguously per lane (1) or lane-coalesced (0).
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                    const uint32_t *ts, uint32_t T1, uint32_t pcap, int contig,
+                    const uint32_t *ts, uint32_t T1, uint32_t pcap, uint32_t bs,
                     const NumOut &o, void *scratch, hipStream_t s);
 // Ultra-sparse expand-sort-compress for all rows with products <= esc_n.
 int launch_esc(int esc_n, const AccArgs &a, const NumOut &o, hipStream_t s);
@@ -169,5 +169,22 @@ int launch_esc(int esc_n, const AccArgs &a, const NumOut &o, hipStream_t s);
 // Per-row sort of (col,val) segments of a device CSR.
 int launch_sort_rows(oc_csr &M, const uint32_t *rows, uint64_t n,
                      hipStream_t s);
+
+// Long rows in column bands, band-major (k_band.hip).
+int band_plan(uint64_t b_cols, uint32_t log2_cols, uint32_t *nbands);
+int launch_band_starts(const DevCsr &B, uint32_t logW, uint32_t nbands, uint32_t *ts,
+                       hipStream_t s);
+// flag[r] = long row; list = flagged rows ascending; pos[rows] = count
+int launch_band_select(const RowInfo &ri, uint64_t rows, uint32_t min_products,
+                       uint32_t band_cols, uint32_t *flag, uint64_t *pos, void *scan_tmp,
+                       uint32_t *list, hipStream_t s);
+int launch_band_count(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logW,
+                      uint32_t nbands, const uint32_t *ts, uint32_t *cnt, uint32_t *counter,
+                      hipStream_t s);
+int launch_band_row_nnz(const uint32_t *rows, uint32_t nrows, uint32_t nbands,
+                        const uint64_t *pref, uint32_t *nnz, hipStream_t s);
+int launch_band_numeric(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logW,
+                        uint32_t nbands, const uint32_t *ts, const uint64_t *pref,
+                        const NumOut &o, uint32_t *counter, hipStream_t s);
 
 }  // namespace oc

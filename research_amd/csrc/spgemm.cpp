@@ -159,6 +159,9 @@ struct oc_spgemm {
     uint32_t tile_logw = 0, tile_t1 = 0, max_a_nnz = 0;
     int tiles_ok = -1;
     void *win_scratch = nullptr;
+    uint32_t band_n = 0, band_nbands = 0, band_logw = 0;
+    uint32_t *band_rows = nullptr, *band_ts = nullptr, *band_counter = nullptr;
+    uint64_t *band_pref = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } plan;
@@ -282,6 +285,12 @@ struct Run {
   uint32_t *tile_ts = nullptr;
   uint32_t tile_logw = 0, tile_t1 = 0;
   int tiles_ok = -1;
+  // long rows in column bands (precise workflow)
+  struct Band {
+    uint32_t n = 0, nbands = 0, logw = 0;
+    uint32_t *flag = nullptr, *rows = nullptr, *ts = nullptr, *cnt = nullptr, *counter = nullptr;
+    uint64_t *pref = nullptr;
+  } band;
   oc_csr *C;
   struct Fam {
     const char *name, *stage;
@@ -339,7 +348,8 @@ struct Run {
     OC_TRY(tiles());
     if (tiles_ok && h->cfg.use_tiled_numeric == 2)
       return launch_num_rank(acc(), rows, n, tile_logw, tile_ts, tile_t1,
-                             (uint32_t)h->cfg.tile_products_cap, h->cfg.rank_layout, o,
+                             (uint32_t)h->cfg.tile_products_cap,
+                             (uint32_t)h->cfg.tile_block, o,
                              win_scratch, s);
     if (tiles_ok)
       return launch_num_tiles(acc(), rows, n, tile_logw, (uint32_t)h->cfg.tile_block, tile_ts,
@@ -381,6 +391,61 @@ struct Run {
     OC_TRY(ar.get(&tmp, bin_temp_bytes(m)));
     OC_TRY(launch_binning(bp, m, *bor, *brows, *boff, tmp, s));
     OC_TRY(readback(hoff, *boff, kBinSlots + 1, s));
+    return OC_OK;
+  }
+
+  // Select the long rows for the band kernels and build their tables.
+  int band_setup() {
+    const uint64_t m = A.rows;
+    band = Band();
+    const uint32_t lw = (uint32_t)std::max(0, c.band_log2_cols);
+    uint32_t nb = 0;
+    if (lw == 0 || !band_plan(B.cols, lw, &nb)) return OC_OK;
+    uint32_t *flag, *list;
+    uint64_t *pos;
+    OC_TRY(ar.get(&flag, m));
+    OC_TRY(ar.get(&pos, m + 1));
+    OC_TRY(ar.get(&list, m));
+    OC_TRY(launch_band_select(ri, m, (uint32_t)std::max(1, c.band_min_products), 1u << lw, flag,
+                              pos, scan_tmp, list, s));
+    uint64_t n = 0;
+    OC_TRY(readback(&n, pos + m, 1, s));
+    if (n == 0) return OC_OK;
+    band.n = (uint32_t)n;
+    band.nbands = nb;
+    band.logw = lw;
+    band.flag = flag;
+    band.rows = list;
+    OC_TRY(ar.get(&band.ts, (uint64_t)(nb + 1) * B.rows));
+    OC_TRY(launch_band_starts(B, lw, nb, band.ts, s));
+    OC_TRY(ar.get(&band.cnt, n * nb));
+    OC_TRY(ar.get(&band.pref, n * nb + 1));
+    OC_TRY(ar.get(&band.counter, 64));
+    mk.mark("band_setup");
+    return OC_OK;
+  }
+
+  // Exact counts of the long rows (per band, then per row) into nnz.
+  int band_symbolic(uint32_t *nnz) {
+    if (!band.n) return OC_OK;
+    const uint64_t cells = (uint64_t)band.n * band.nbands;
+    OC_TRY(launch_band_count(acc(), band.rows, band.n, band.logw, band.nbands, band.ts,
+                             band.cnt, band.counter, s));
+    uint8_t *tmp;
+    OC_TRY(ar.get(&tmp, scan_temp_bytes(cells)));
+    OC_TRY(launch_scan_u32(band.cnt, band.pref, cells, tmp, s));
+    OC_TRY(launch_band_row_nnz(band.rows, band.n, band.nbands, band.pref, nnz, s));
+    fam("band_count", "symbolic_band", false, band.rows, band.n);
+    mk.mark("symbolic_band");
+    return OC_OK;
+  }
+
+  int band_numeric(const NumOut &o) {
+    if (!band.n) return OC_OK;
+    OC_TRY(launch_band_numeric(acc(), band.rows, band.n, band.logw, band.nbands, band.ts,
+                               band.pref, o, band.counter, s));
+    fam("band_numeric", "numeric_band", true, band.rows, band.n);
+    mk.mark("numeric_band");
     return OC_OK;
   }
 
@@ -565,6 +630,8 @@ int Run::run_precise() {
   bp.use_compaction = d.use_est_symbolic;
   bp.expand_factor = c.symbolic_expansion_coe;
   bp.avg_compaction = d.safe_comp;
+  OC_TRY(band_setup());
+  bp.skip_flag = band.flag;
   uint8_t *bor = nullptr;
   uint32_t *brows = nullptr, *boff = nullptr;
   uint32_t hoff[kBinSlots + 1];
@@ -611,6 +678,7 @@ int Run::run_precise() {
     }
     mk.mark("symbolic_overflow");
   }
+  OC_TRY(band_symbolic(nnz));
   OC_TRY(alloc_c_from_counts(nnz));
   // numeric binning on exact counts, SpGEMM.cuh:1247-1283
   BinParams nb;
@@ -623,6 +691,7 @@ int Run::run_precise() {
   nb.expand_factor = c.numeric_expansion_coe;
   nb.use_largest = d.hash_largest;
   nb.hybrid_max = (uint32_t)(m / 10);
+  nb.skip_flag = band.flag;
   uint8_t *bor2 = nullptr;
   uint32_t *brows2 = nullptr, *boff2 = nullptr;
   OC_TRY(binning(nb, &bor2, &brows2, &boff2, hoff));
@@ -638,6 +707,7 @@ int Run::run_precise() {
   o.ovf_count = ovf + 1;
   o.nnz_hint = nnz;
   OC_TRY(numeric_bins(brows2, hoff, o, true));
+  OC_TRY(band_numeric(o));
   // exact capacities cannot overflow; recount defensively if one did
   uint32_t n = 0;
   OC_TRY(readback(&n, ovf + 1, 1, s));
@@ -657,6 +727,13 @@ int Run::run_precise() {
     p.tiles_ok = tiles_ok;
     p.win_scratch = win_scratch;
     p.max_a_nnz = h->max_a_nnz;
+    p.band_n = band.n;
+    p.band_nbands = band.nbands;
+    p.band_logw = band.logw;
+    p.band_rows = band.rows;
+    p.band_ts = band.ts;
+    p.band_counter = band.counter;
+    p.band_pref = band.pref;
     p.valid = n == 0;  // a recount would make the numeric phase two-step
   }
   return OC_OK;
@@ -1128,9 +1205,17 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   r.tile_t1 = p.tile_t1;
   r.tiles_ok = p.tiles_ok;
   r.win_scratch = p.win_scratch;
+  r.band.n = p.band_n;
+  r.band.nbands = p.band_nbands;
+  r.band.logw = p.band_logw;
+  r.band.rows = p.band_rows;
+  r.band.ts = p.band_ts;
+  r.band.counter = p.band_counter;
+  r.band.pref = p.band_pref;
   const uint32_t keep_max = h->max_a_nnz;
   h->max_a_nnz = p.max_a_nnz;
   int rc = r.numeric_bins(p.brows, p.hoff, p.o, true);
+  if (rc == OC_OK) rc = r.band_numeric(p.o);
   h->max_a_nnz = keep_max;
   if (rc == OC_OK && !r.ar.ptrs.empty()) {
     set_error("internal: a replay allocated memory");

@@ -68,7 +68,8 @@ class _Config(C.Structure):
         ("use_learned_binning", C.c_int32), ("learned_model_file", C.c_char * 512),
         ("use_tiled_numeric", C.c_int32), ("tile_log2_cols", C.c_int32),
         ("tile_block", C.c_int32), ("tile_products_cap", C.c_int32),
-        ("rank_layout", C.c_int32), ("learned_quantile", C.c_double),
+        ("learned_quantile", C.c_double),
+        ("band_log2_cols", C.c_int32), ("band_min_products", C.c_int32),
     ]
 
 

@@ -79,14 +79,34 @@ def test_precise_dense_and_hash_bins(gpu):
     log(f"bins: {st.get('binning_1')} / {st.get('binning_2')}")
 
 
-@pytest.mark.parametrize("tiled", [2, 1, 0])
-def test_big_rows_tiled_and_windowed(gpu, tiled):
+@pytest.mark.parametrize("tiled,block", [(2, 1024), (2, 512), (2, 256), (1, 1024), (0, 1024)])
+def test_big_rows_tiled_and_windowed(gpu, tiled, block):
     """Rows beyond the LDS hash/dense kernels (numeric bin 13) through the
-    tiled kernel and through the cursor-windowed kernel."""
-    log(f"case: rmat 2^15 deg 32 f64, big rows, use_tiled_numeric={tiled}")
+    rank-tile kernel (three workgroup shapes), the dense-tile kernel and
+    the cursor-windowed kernel."""
+    log(f"case: rmat 2^15 deg 32 f64, big rows, use_tiled_numeric={tiled} block={block}")
     A = gpu.CSR.rmat(15, 32 << 15, seed=5)
-    cfg = gpu.Config(use_tiled_numeric=tiled, track_stage_times=1)
+    cfg = gpu.Config(use_tiled_numeric=tiled, tile_block=block, band_log2_cols=0,
+                     track_stage_times=1)
     C, st = _run(gpu, A, A, cfg)
     assert st["binning_2"][14] - st["binning_2"][13] > 0, "no rows reached the big-row bin"
-    worst, plain = check_product(A, A, C, TOL64, f"bigrows tiled={tiled}")
-    log(f"tiled={tiled}: big rows {st['binning_2'][14] - st['binning_2'][13]} err {worst:.2e}")
+    worst, plain = check_product(A, A, C, TOL64, f"bigrows tiled={tiled} block={block}")
+    log(f"tiled={tiled} block={block}: big rows {st['binning_2'][14] - st['binning_2'][13]} "
+        f"err {worst:.2e}")
+
+
+@pytest.mark.parametrize("log_n,deg,log_w,min_products,vtype",
+                         [(15, 32, 14, 16384, 0), (15, 32, 12, 2000, 0), (16, 16, 15, 4000, 0),
+                          (15, 32, 13, 1000, 1)])
+def test_band_long_rows(gpu, log_n, deg, log_w, min_products, vtype):
+    """Long rows of the precise workflow through the band-major kernels
+    (count, then numeric), with other rows in the usual bins (a row takes
+    the band path when it is wider than one band)."""
+    log(f"case: rmat 2^{log_n} deg {deg} vtype {vtype}, bands 2^{log_w}, min products {min_products}")
+    A = gpu.CSR.rmat(log_n, deg << log_n, vtype=vtype, seed=5)
+    cfg = gpu.Config(band_log2_cols=log_w, band_min_products=min_products, track_stage_times=1)
+    C, st = _run(gpu, A, A, cfg, 0)
+    skipped = st["binning_2"][15] - st["binning_2"][14]
+    assert skipped > 0 and "band_numeric" in st["kernels"], "no rows took the band path"
+    worst, plain = check_product(A, A, C, TOL64 if vtype == 0 else TOL32, f"bands 2^{log_w}")
+    log(f"bands 2^{log_w}: {skipped} long rows, err {worst:.2e}")
