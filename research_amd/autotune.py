@@ -1,0 +1,141 @@
+"""Autotuner for the SpGEMM path's bin thresholds, table use and block shapes.
+
+Coordinate descent over the configuration knobs that change how rows are
+binned and which kernels and workgroup shapes take them:
+
+  symbolic_expansion_coe, numeric_expansion_coe   bin thresholds (sizes
+                                                  are scaled by these before
+                                                  a row is binned, so they set
+                                                  the table load factor)
+  use_tiled_numeric, tile_log2_cols, tile_block   long-row kernel, tile width
+                                                  and workgroup shape
+  band_log2_cols                                  band-major long rows (0 off)
+
+Each candidate runs the workload a few times; the score is the median of
+the library's own stage-timer total.  A candidate whose output structure
+(per-row column-set hashes) differs from the default's is rejected.  The
+result is a config JSON (same keys as the reference's config files) that
+Config.load and the spgemm CLI read.
+
+    python -m research_amd.autotune --workload rmat --log-n 18 --out tuned.json
+    python -m research_amd.autotune --workload batched --count 256 --out tuned.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import ocean
+
+SPACE = {
+    "symbolic_expansion_coe": [1.25, 1.0, 1.5],
+    "numeric_expansion_coe": [1.5, 1.25, 2.0],
+    "use_tiled_numeric": [2, 1],
+    "tile_log2_cols": [14, 13, 15],
+    "tile_block": [1024, 512],
+    "band_log2_cols": [0, 14, 13],
+}
+
+
+class _Product:
+    """One product C = A*A through SpGEMM."""
+
+    def __init__(self, A):
+        self.dA = A.upload()
+
+    def run(self, cfg: dict):
+        sp = ocean.SpGEMM(ocean.Config(**cfg))
+        C = sp.run(self.dA, self.dA)
+        ms = sp.timer_ms("total")
+        h = C.row_set_hash()
+        C.free()
+        return ms, h
+
+
+class _Batched:
+    """A batch of products through one block-diagonal run."""
+
+    def __init__(self, mats):
+        self.dev = [M.upload() for M in mats]
+
+    def run(self, cfg: dict):
+        b = ocean.Batch(ocean.Config(**cfg))
+        b.set_inputs(self.dev)
+        b.run()
+        ms = b.timer_ms("total")
+        view, _, _ = b.result()
+        return ms, view.row_set_hash()
+
+
+def _score(w, cfg, reps):
+    times, h = [], None
+    for _ in range(reps):
+        ms, h = w.run(cfg)
+        times.append(ms)
+    return float(np.median(times)), h
+
+
+def tune(w, reps: int = 3, passes: int = 2, log=print) -> dict:
+    cur = {k: v[0] for k, v in SPACE.items()}
+    best_ms, ref_hash = _score(w, cur, reps)
+    history = [{"config": dict(cur), "ms": best_ms}]
+    if log:
+        log(f"[autotune] default: {best_ms:.2f} ms", flush=True)
+    for p in range(passes):
+        changed = False
+        for knob, values in SPACE.items():
+            for v in values:
+                if v == cur[knob]:
+                    continue
+                cand = dict(cur, **{knob: v})
+                try:
+                    ms, h = _score(w, cand, reps)
+                except ocean.OceanError as e:
+                    history.append({"config": cand, "error": str(e)})
+                    continue
+                ok = np.array_equal(h, ref_hash)
+                history.append({"config": cand, "ms": ms, "same_structure": bool(ok)})
+                if log:
+                    log(f"[autotune] pass {p} {knob}={v}: {ms:.2f} ms"
+                        f"{'' if ok else ' (structure differs: rejected)'}", flush=True)
+                if ok and ms < best_ms * 0.98:
+                    best_ms, cur, changed = ms, cand, True
+        if not changed:
+            break
+    return {"config": cur, "ms": best_ms, "default_ms": history[0]["ms"],
+            "speedup": round(history[0]["ms"] / best_ms, 3), "evaluations": len(history),
+            "history": history}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m research_amd.autotune")
+    ap.add_argument("--workload", choices=["rmat", "batched"], default="rmat")
+    ap.add_argument("--log-n", type=int, default=18)
+    ap.add_argument("--degree", type=int, default=16)
+    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    t0 = time.time()
+    if a.workload == "rmat":
+        w = _Product(ocean.CSR.rmat(a.log_n, a.degree << a.log_n, vtype=0, seed=1))
+        desc = f"R-MAT n=2^{a.log_n}, degree {a.degree}, f64"
+    else:
+        from . import workloads
+        w = _Batched(workloads.batched_products(ocean, a.count))
+        desc = f"{a.count} batched products (ER / banded / R-MAT, f32)"
+    res = tune(w, a.reps)
+    res["workload"] = desc
+    res["seconds"] = round(time.time() - t0, 1)
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(res["config"], f, indent=1)
+    print(json.dumps({k: v for k, v in res.items() if k != "history"}), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -984,12 +984,25 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       }
       __syncthreads();
     };
+    // Tile starts of this thread's A entry.  Single-chunk rows walk their
+    // tiles in order, so a tile's end is the next tile's start and the end
+    // after it is loaded one tile ahead.
+    auto tsv = [&](uint32_t k, uint32_t tt) {
+      return ts[(uint64_t)k * T1 + min(T1 - 1, tt << step)];
+    };
+    uint32_t ts_cur = 0, ts_end = 0;
     auto bounds = [&](uint32_t t, uint32_t nc, uint32_t &P) {
       uint32_t len = 0, st = 0;
       if (tid < nc) {
-        const uint64_t base = (uint64_t)sk[tid] * T1;
-        st = ts[base + (t << step)];
-        len = ts[base + min(T1 - 1, (t + 1) << step)] - st;
+        if (one_chunk) {
+          st = ts_cur;
+          len = ts_end - ts_cur;
+          ts_cur = ts_end;
+          if (t + 1 <= t_end) ts_end = tsv(sk[tid], t + 2);
+        } else {
+          st = tsv(sk[tid], t);
+          len = tsv(sk[tid], t + 1) - st;
+        }
       }
       const uint32_t ex = block_excl_sum(len, hdr + 8, &P);
       my_ex = ex;
@@ -1032,7 +1045,13 @@ __global__ __launch_bounds__(BS) void k_num_rank(
         }
       }
     };
-    if (one_chunk) stage(a0, nA);
+    if (one_chunk) {
+      stage(a0, nA);
+      if (tid < nA) {
+        ts_cur = tsv(sk[tid], t_begin);
+        ts_end = tsv(sk[tid], t_begin + 1);
+      }
+    }
     for (uint32_t t = t_begin; t <= t_end; ++t) {
       const uint64_t wbase = (uint64_t)t << lw;
       // ---- pass 1: mark the output columns
