@@ -99,7 +99,8 @@ def adam(p, m1, m2, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
 
 # ------------------------------------------------------------- features
 def features(A, B) -> np.ndarray:
-    """The sixteen per-row features of C = A*B (FP64)."""
+    """The sixteen per-row features of C = A*B (FP64), all properties of
+    the row alone."""
     an = ora.analysis(A, B)
     m = A.rows
     P = an["products"].astype(np.float64)
@@ -114,7 +115,7 @@ def features(A, B) -> np.ndarray:
         uni = np.where(span > 0, span * (1 - np.exp(-P / np.where(span > 0, span, 1))), 0.0)
         bal = np.where((nA > 0) & (mb > 0), P / np.where((nA > 0) & (mb > 0), nA * mb, 1), 0.0)
     lg = lambda v: np.log2(1.0 + v) / 16.0  # noqa: E731
-    b_nnz = len(B.col_ids)
+    safe = np.where(span > 0, span, 1.0)
     out = np.zeros((m, F))
     out[:, 0] = lg(P)
     out[:, 1] = lg(nA)
@@ -125,12 +126,12 @@ def features(A, B) -> np.ndarray:
     out[:, 6] = lg(uni)
     out[:, 7] = lg(P) - lg(mb)
     out[:, 8] = bal
-    out[:, 9] = span / B.cols if B.cols else 0.0
-    out[:, 10] = np.where(span > 0, lo / B.cols, 0.0) if B.cols else 0.0
-    out[:, 11] = np.arange(m) / m
-    out[:, 12] = np.log2(1.0 + (b_nnz / B.rows if B.rows else 0.0)) / 16.0
-    out[:, 13] = np.log2(1.0 + len(A.col_ids) / m) / 16.0
-    out[:, 14] = np.log2(1.0 + B.cols) / 16.0
+    out[:, 9] = np.where(span > 0, np.minimum(P / safe, 4.0) * 0.25, 0.0)
+    out[:, 10] = np.where(span > 0, mb / safe, 0.0)
+    out[:, 11] = lg(ub) - lg(mb)
+    out[:, 12] = lg(uni) - lg(mb)
+    out[:, 13] = lg(ub) - lg(uni)
+    out[:, 14] = 0.0
     out[:, 15] = 1.0
     return out
 

@@ -405,11 +405,12 @@ __global__ void k_mlp_adam(float *__restrict__ master, float *__restrict__ m1,
 
 // --------------------------------------------------- features and labels
 // Sixteen features per row of C = A*B from the row analysis (all O(1)
-// after the 1/16 scaling of the base-2 logarithms).
+// after the 1/16 scaling of the base-2 logarithms).  Every feature is a
+// property of the row alone, so a row scores the same in a product of its
+// own and inside a batch of many products.
 struct FeatArgs {
   const uint32_t *a_rp, *products, *max_b, *avg_b, *left, *right;
   uint64_t rows;
-  float inv_bcols, inv_rows, g_bavg, g_aavg, g_bcols;
 };
 
 __global__ void k_mlp_features(FeatArgs fa, float *__restrict__ feat) {
@@ -423,22 +424,23 @@ __global__ void k_mlp_features(FeatArgs fa, float *__restrict__ feat) {
     const float ub = fminf(P, span);
     const float uni = span > 0.f ? span * (1.f - expf(-P / span)) : 0.f;
     const float lP = log2f(1.f + P), lmb = log2f(1.f + mb);
+    const float lub = log2f(1.f + ub), luni = log2f(1.f + uni);
     float *f = feat + r * kF;
     f[0] = lP * 0.0625f;
     f[1] = log2f(1.f + nA) * 0.0625f;
     f[2] = lmb * 0.0625f;
     f[3] = log2f(1.f + ab) * 0.0625f;
     f[4] = log2f(1.f + span) * 0.0625f;
-    f[5] = log2f(1.f + ub) * 0.0625f;
-    f[6] = log2f(1.f + uni) * 0.0625f;
+    f[5] = lub * 0.0625f;
+    f[6] = luni * 0.0625f;
     f[7] = (lP - lmb) * 0.0625f;
     f[8] = (nA > 0.f && mb > 0.f) ? P / (nA * mb) : 0.f;
-    f[9] = span * fa.inv_bcols;
-    f[10] = span > 0.f ? (float)lo * fa.inv_bcols : 0.f;
-    f[11] = (float)r * fa.inv_rows;
-    f[12] = fa.g_bavg;
-    f[13] = fa.g_aavg;
-    f[14] = fa.g_bcols;
+    f[9] = span > 0.f ? fminf(P / span, 4.f) * 0.25f : 0.f;  // product density
+    f[10] = span > 0.f ? mb / span : 0.f;                     // longest B row / span
+    f[11] = (lub - lmb) * 0.0625f;   // room between the bounds mb <= nnz <= ub
+    f[12] = (luni - lmb) * 0.0625f;
+    f[13] = (lub - luni) * 0.0625f;
+    f[14] = 0.f;
     f[15] = 1.f;
   }
 }
@@ -488,11 +490,7 @@ int launch_mlp_features(const DevCsr &A, const DevCsr &B, const RowInfo &ri, flo
   fa.left = ri.left;
   fa.right = ri.right;
   fa.rows = A.rows;
-  fa.inv_bcols = B.cols ? 1.f / (float)B.cols : 0.f;
-  fa.inv_rows = 1.f / (float)A.rows;
-  fa.g_bavg = (float)(std::log2(1.0 + (B.rows ? (double)B.nnz / (double)B.rows : 0.0)) / 16.0);
-  fa.g_aavg = (float)(std::log2(1.0 + (double)A.nnz / (double)A.rows) / 16.0);
-  fa.g_bcols = (float)(std::log2(1.0 + (double)B.cols) / 16.0);
+  (void)B;
   hipLaunchKernelGGL(k_mlp_features, dim3(grid_1d(A.rows, 256)), dim3(256), 0, s, fa, feat);
   LAUNCH_CHECK();
   return OC_OK;

@@ -58,6 +58,14 @@ struct Arena {
     ptrs.push_back((void *)*p);
     return OC_OK;
   }
+  void free(void *p) {
+    for (size_t i = 0; i < ptrs.size(); ++i)
+      if (ptrs[i] == p) {
+        (void)hipFreeAsync(p, s);
+        ptrs.erase(ptrs.begin() + (long)i);
+        return;
+      }
+  }
   void release() {
     for (void *p : ptrs) (void)hipFreeAsync(p, s);
     ptrs.clear();
@@ -128,6 +136,8 @@ struct oc_spgemm {
   bool have_bin1 = false, have_bin2 = false;
   uint64_t total_est = 0;
   uint32_t overflow_rows = 0, sym_overflow_rows = 0, precise_recount = 0;
+  bool est_fallback = false;  // estimation workflow fell back to precise
+  uint64_t est_spare_need = 0;
   int vtype = OC_F64;
   double total_ms = 0;
   std::map<std::string, double> stage_ms;
@@ -840,9 +850,15 @@ int Run::run_est() {
     unsigned long long hu = 0;
     OC_TRY(readback(&hu, used, 1, s));
     if (h->total_est + hu > cap_total) {
-      set_error("Not enough overflow buffer allocated. Try to increase "
-                "hll_numeric_malloc_expansion_coe (default at 1.3).");
-      return OC_ERR_CAPACITY;
+      // The spare region (SpGEMM.cuh:1583-1587 raises here) cannot hold the
+      // rows the estimates missed: recompute this product with exact counts.
+      h->est_fallback = true;
+      h->est_spare_need = hu;
+      ar.free(tc);
+      ar.free(tv);
+      h->d.ana2_type = 0;
+      mk.mark("numeric_overflow");
+      return run_precise();
     }
     NumOut o2 = o;
     o2.off = src_off;
@@ -947,6 +963,12 @@ std::string stats_json(const oc_spgemm *h) {
   }
   if (workflow == 3) j.u64("fat_runs_precise_kernels", 1);
   j.boolean("learned_estimates", h->learned_run);
+  if (h->est_fallback) {
+    j.boolean("estimation_fell_back_to_precise", true);
+    j.u64("estimation_overflow_rows", h->overflow_rows);
+    j.u64("estimation_spare_needed", h->est_spare_need);
+    j.u64("estimation_total_est", h->total_est);
+  }
   j.close();
   if (workflow <= 1) {
     j.open("sampling");
@@ -1101,6 +1123,8 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
   h->have_bin1 = h->have_bin2 = false;
   h->total_est = 0;
   h->overflow_rows = h->sym_overflow_rows = h->precise_recount = 0;
+  h->est_fallback = false;
+  h->est_spare_need = 0;
   h->stage_ms.clear();
   std::memset(dC, 0, sizeof(*dC));
   Run r(h, s, dC);

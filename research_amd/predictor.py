@@ -149,13 +149,17 @@ def evaluate(model: str, log_n: int = 21, degree: int = 32, panel_products: floa
     ovf = est_total = nnz_total = products = 0
     same = True
     panels = panels_by_products(A, panel_products)
-    for r0, r1 in panels:
+    for i, (r0, r1) in enumerate(panels):
         P = _panel(dA, r0, r1, ro)
+        if log:
+            log(f"[predictor] panel {i}: rows {r0}..{r1}, plain path", flush=True)
         C1 = plain.run(P, dA)
         t_plain += plain.timer_ms("total")
         h1, o1 = C1.row_set_hash(), C1.row_offsets()
         st1 = plain.get_stats()
         C1.free()
+        if log:
+            log(f"[predictor] panel {i}: learned path", flush=True)
         C2 = learned.run(P, dA)
         t_learned += learned.timer_ms("total")
         st2 = learned.get_stats()

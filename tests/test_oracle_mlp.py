@@ -96,12 +96,13 @@ def test_features_brute_force():
     P, nA, mb, ab, lo, hi = 10, 3, 7, 3, 0, 9
     span = hi - lo + 1
     lg = lambda v: math.log2(1 + v) / 16  # noqa: E731
-    want = [lg(P), lg(nA), lg(mb), lg(ab), lg(span), lg(min(P, span)),
-            lg(span * (1 - math.exp(-P / span))), lg(P) - lg(mb), P / (nA * mb),
-            span / 10, lo / 10, 0.0, lg(11 / 4), lg(6 / 4), lg(10), 1.0]
+    ub, uni = min(P, span), span * (1 - math.exp(-P / span))
+    want = [lg(P), lg(nA), lg(mb), lg(ab), lg(span), lg(ub), lg(uni), lg(P) - lg(mb),
+            P / (nA * mb), min(P / span, 4) / 4, mb / span, lg(ub) - lg(mb),
+            lg(uni) - lg(mb), lg(ub) - lg(uni), 0.0, 1.0]
     assert np.allclose(f[0], want, rtol=1e-12, atol=1e-15)
-    # empty A row: span 0 (sentinels), everything but the globals is 0
-    assert np.allclose(f[1, :11], 0.0) and f[1, 11] == 0.25
+    # empty A row: span 0 (sentinels), everything but the bias is 0
+    assert np.allclose(f[1, :15], 0.0) and f[1, 15] == 1.0
     # row 2 references B row 2 twice: P = 2, span 1, min(P, span) = 1
     assert f[2, 4] == pytest.approx(lg(1)) and f[2, 5] == pytest.approx(lg(1))
 
