@@ -101,9 +101,13 @@ def run_batched(ra, ocean, count: int, steps: int, model_path: str | None):
         st = b.get_stats()
         out.update({"learned_batched_gflops": round(2 * products / learned_s / 1e9, 3),
                     "learned_batched_latency_ms": round(learned_s * 1e3, 3),
-                    "learned_overflow_rows": st.get("estimation", {}).get("overflow_rows"),
+                    "learned_workflow": st["decision"]["workflow"],
+                    "learned_products_per_capacity":
+                        st["decision"].get("learned_products_per_capacity"),
+                    "learned_overflow_rows": (st.get("estimation") or {}).get("overflow_rows"),
                     "learned_same_structure": bool(np.array_equal(view.row_set_hash(), ref_hash)),
-                    "learned_batched_vs_plain": round(plain_s / learned_s, 2)})
+                    "learned_batched_vs_plain": round(plain_s / learned_s, 2),
+                    "learned_batched_vs_batched": round(batched_s / learned_s, 3)})
         ocean._check(ocean.lib.oc_spgemm_set_model(ocean.lib.oc_batch_handle(b.h), None, 0.99))
     # value-only graph replays of the planned batch
     b.plan()

@@ -130,6 +130,10 @@ typedef struct oc_config {
    * with at least band_min_products products and a wider span. */
   int32_t band_log2_cols;
   int32_t band_min_products;
+  /* Learned predictor: take the estimation workflow when the products are
+   * at least this many times the sum of the predicted capacities, else the
+   * precise workflow (default 1.5). */
+  double learned_min_compression;
 } oc_config;
 
 /* Replaces: Config's default member initialisers (include/Utils.h:22-107). */

@@ -89,7 +89,7 @@ extern "C" int oc_analysis(const oc_csr *dA, const oc_csr *dB, uint32_t *num_pro
   Tmp sc(s);
   OC_TRY(sc.get(16 + scan_temp_bytes(dA->rows + 1)));
   uint32_t *scal = (uint32_t *)sc.p;
-  OC_TRY(launch_analysis(dev_view(dA), dev_view(dB), ri, scal, scal + 1, s));
+  OC_TRY(launch_analysis(dev_view(dA), dev_view(dB), ri, scal, scal + 1, nullptr, s));
   if (products_psum)
     OC_TRY(launch_scan_u32(num_products, products_psum, dA->rows, (uint8_t *)sc.p + 16, s));
   if (max_a_nnz) HIP_TRY(hipMemcpyAsync(max_a_nnz, scal, 4, hipMemcpyDeviceToDevice, s));
@@ -221,7 +221,7 @@ extern "C" int oc_symbolic(const oc_config *cfg, const oc_csr *dA, const oc_csr 
   uint32_t *scal = nullptr;
   HIP_TRY(hipMallocAsync((void **)&scal, 256, s));
   const DevCsr A = dev_view(dA), B = dev_view(dB);
-  int rc = launch_analysis(A, B, ri, scal, scal + 1, s);
+  int rc = launch_analysis(A, B, ri, scal, scal + 1, nullptr, s);
   uint32_t hs[2] = {0, 0};
   if (rc == OC_OK) {
     HIP_TRY(hipMemcpyAsync(hs, scal, 8, hipMemcpyDeviceToHost, s));

@@ -65,6 +65,7 @@ extern "C" void oc_config_default(oc_config *c) {
   c->learned_quantile = 0.99;
   c->band_log2_cols = 0;
   c->band_min_products = 16384;
+  c->learned_min_compression = 1.5;
 }
 
 namespace {
@@ -282,6 +283,7 @@ extern "C" int oc_config_from_json(oc_config *c, const char *text) {
     D("learned_quantile", learned_quantile)
     I("band_log2_cols", band_log2_cols)
     I("band_min_products", band_min_products)
+    D("learned_min_compression", learned_min_compression)
 #undef B
 #undef I
 #undef D

@@ -668,7 +668,7 @@ extern "C" int oc_mlp_features(const oc_csr *dA, const oc_csr *dB, float *feat, 
                  dA->col_ids, dA->values, dA->vtype};
   const DevCsr B{dB->rows, dB->cols, dB->nnz, (const uint32_t *)dB->row_offsets,
                  dB->col_ids, dB->values, dB->vtype};
-  int rc = launch_analysis(A, B, ri, buf + 5 * m, buf + 5 * m + 1, s);
+  int rc = launch_analysis(A, B, ri, buf + 5 * m, buf + 5 * m + 1, nullptr, s);
   if (rc == OC_OK) rc = launch_mlp_features(A, B, ri, feat, s);
   (void)hipFreeAsync(buf, s);
   OC_TRY(rc);
@@ -800,7 +800,7 @@ extern "C" int oc_mlp_estimates(const oc_mlp *m, const oc_csr *dA, const oc_csr 
                  dA->col_ids, dA->values, dA->vtype};
   const DevCsr B{dB->rows, dB->cols, dB->nnz, (const uint32_t *)dB->row_offsets,
                  dB->col_ids, dB->values, dB->vtype};
-  int rc = launch_analysis(A, B, ri, buf + 5 * rows, buf + 5 * rows + 1, s);
+  int rc = launch_analysis(A, B, ri, buf + 5 * rows, buf + 5 * rows + 1, nullptr, s);
   if (rc == OC_OK) rc = launch_mlp_features(A, B, ri, feat, s);
   if (rc == OC_OK) rc = launch_mlp_estimates(m, feat, rows, ri, quantile, est, s);
   (void)hipFreeAsync(feat, s);

@@ -90,38 +90,76 @@ const Tables &tables() {
 // Per A row: products = sum of referenced B row lengths, longest B row, the
 // average, and the min first / max last column of the referenced non-empty
 // B rows; grid-wide max A row length and max products
-// (kernels/Analysis.cuh:31-105).  G lanes per row.
+// (kernels/Analysis.cuh:31-105).
+//
+// A first pass writes one 16-byte record per B row {length, first column,
+// last column}, so each A entry costs one gather of a compact table (B.rows
+// x 16 B, cache-resident for the block-local column ranges of batched
+// inputs) instead of three dependent gathers into B's offsets and columns.
+// Rows of A up to kLongA entries take G lanes each; longer rows are listed
+// and taken by a whole workgroup each in a second pass, so one hub row does
+// not serialise its group for thousands of dependent loads.
+constexpr uint32_t kLongAIters = 32;  // iterations per lane before a row is "long"
+constexpr int kLongBlock = 256;
+
+__global__ __launch_bounds__(256) void k_brow_info(const uint32_t *__restrict__ b_rp,
+                                                   const uint32_t *__restrict__ b_ci,
+                                                   uint32_t b_rows, uint4 *__restrict__ info) {
+  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < b_rows; k += gridDim.x * 256) {
+    const uint32_t bs = b_rp[k], be = b_rp[k + 1];
+    uint4 v = make_uint4(0u, 0xFFFFFFFFu, 0u, 0u);
+    if (be > bs) v = make_uint4(be - bs, b_ci[bs], b_ci[be - 1], 0u);
+    info[k] = v;
+  }
+}
+
+struct AnaAcc {
+  uint32_t prod = 0, mb = 0, lo = 0xFFFFFFFFu, hi = 0;
+  __device__ __forceinline__ void add(const uint4 v) {
+    // an empty B row is {0, ~0, 0}: neutral for every field
+    prod += v.x;
+    mb = max(mb, v.x);
+    lo = min(lo, v.y);
+    hi = max(hi, v.z);
+  }
+};
+
 template <int G>
 __global__ __launch_bounds__(256) void k_analysis(
     const uint32_t *__restrict__ a_rp, const uint32_t *__restrict__ a_ci,
-    uint32_t a_rows, const uint32_t *__restrict__ b_rp,
-    const uint32_t *__restrict__ b_ci, uint32_t *__restrict__ products,
+    uint32_t a_rows, const uint4 *__restrict__ info, uint32_t *__restrict__ products,
     uint32_t *__restrict__ max_b, uint32_t *__restrict__ avg_b,
     uint32_t *__restrict__ left, uint32_t *__restrict__ right,
-    uint32_t *max_a_nnz, uint32_t *max_products) {
+    uint32_t *max_a_nnz, uint32_t *max_products, uint32_t *__restrict__ long_rows,
+    uint32_t *long_count) {
   constexpr int RPB = 256 / G;
   const int lig = threadIdx.x % G;
   uint32_t my_max_nnz = 0, my_max_prod = 0;
   for (uint32_t r = blockIdx.x * RPB + threadIdx.x / G; r < a_rows;
        r += gridDim.x * RPB) {
     const uint32_t s = a_rp[r], e = a_rp[r + 1];
-    uint32_t prod = 0, mb = 0, lo = 0xFFFFFFFFu, hi = 0;
-    for (uint32_t j = s + lig; j < e; j += G) {
-      const uint32_t k = a_ci[j];
-      const uint32_t bs = b_rp[k], be = b_rp[k + 1];
-      if (be <= bs) continue;
-      const uint32_t c = be - bs;
-      prod += c;
-      mb = max(mb, c);
-      lo = min(lo, b_ci[bs]);
-      hi = max(hi, b_ci[be - 1]);
+    const uint32_t n = e - s;
+    if (n > kLongAIters * G) {  // uniform over the group
+      if (lig == 0) {
+        long_rows[atomicAdd(long_count, 1u)] = r;
+        my_max_nnz = max(my_max_nnz, n);
+      }
+      continue;
     }
-    prod = group_sum(prod, G);
-    mb = group_max(mb, G);
-    lo = group_min(lo, G);
-    hi = group_max(hi, G);
+    AnaAcc acc;
+    uint32_t j = s + lig;
+    for (; j + G < e; j += 2 * G) {  // two independent gathers in flight
+      const uint32_t k0 = a_ci[j], k1 = a_ci[j + G];
+      const uint4 v0 = info[k0], v1 = info[k1];
+      acc.add(v0);
+      acc.add(v1);
+    }
+    if (j < e) acc.add(info[a_ci[j]]);
+    const uint32_t prod = group_sum(acc.prod, G);
+    const uint32_t mb = group_max(acc.mb, G);
+    const uint32_t lo = group_min(acc.lo, G);
+    const uint32_t hi = group_max(acc.hi, G);
     if (lig == 0) {
-      const uint32_t n = e - s;
       products[r] = prod;
       max_b[r] = mb;
       avg_b[r] = n == 0 ? 0u : prod / n;
@@ -131,35 +169,125 @@ __global__ __launch_bounds__(256) void k_analysis(
       my_max_prod = max(my_max_prod, prod);
     }
   }
+  // one atomic per workgroup: same-address atomics serialise in L2, so
+  // one per wave over millions of rows would dominate the kernel
+  __shared__ uint32_t red[2][256 / kWave];
   my_max_nnz = group_max(my_max_nnz, kWave);
   my_max_prod = group_max(my_max_prod, kWave);
   if (lane_id() == 0) {
+    red[0][threadIdx.x / kWave] = my_max_nnz;
+    red[1][threadIdx.x / kWave] = my_max_prod;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 256 / kWave; ++w) {
+      my_max_nnz = max(my_max_nnz, red[0][w]);
+      my_max_prod = max(my_max_prod, red[1][w]);
+    }
     if (max_a_nnz) atomicMax(max_a_nnz, my_max_nnz);
     if (max_products) atomicMax(max_products, my_max_prod);
   }
 }
 
+// The listed long rows, one workgroup per row (grid-stride over the list).
+__global__ __launch_bounds__(kLongBlock) void k_analysis_long(
+    const uint32_t *__restrict__ a_rp, const uint32_t *__restrict__ a_ci,
+    const uint4 *__restrict__ info, uint32_t *__restrict__ products,
+    uint32_t *__restrict__ max_b, uint32_t *__restrict__ avg_b,
+    uint32_t *__restrict__ left, uint32_t *__restrict__ right, uint32_t *max_products,
+    const uint32_t *__restrict__ long_rows, const uint32_t *long_count) {
+  constexpr int NW = kLongBlock / kWave;
+  __shared__ uint4 part[NW];
+  const uint32_t nlong = *long_count;
+  const int wid = threadIdx.x / kWave;
+  for (uint32_t i = blockIdx.x; i < nlong; i += gridDim.x) {
+    const uint32_t r = long_rows[i];
+    const uint32_t s = a_rp[r], e = a_rp[r + 1];
+    AnaAcc acc;
+    uint32_t j = s + threadIdx.x;
+    for (; j + 3 * kLongBlock < e; j += 4 * kLongBlock) {
+      const uint32_t k0 = a_ci[j], k1 = a_ci[j + kLongBlock], k2 = a_ci[j + 2 * kLongBlock],
+                     k3 = a_ci[j + 3 * kLongBlock];
+      const uint4 v0 = info[k0], v1 = info[k1], v2 = info[k2], v3 = info[k3];
+      acc.add(v0);
+      acc.add(v1);
+      acc.add(v2);
+      acc.add(v3);
+    }
+    for (; j < e; j += kLongBlock) acc.add(info[a_ci[j]]);
+    const uint32_t prod = group_sum(acc.prod, kWave);
+    const uint32_t mb = group_max(acc.mb, kWave);
+    const uint32_t lo = group_min(acc.lo, kWave);
+    const uint32_t hi = group_max(acc.hi, kWave);
+    if (lane_id() == 0) part[wid] = make_uint4(prod, mb, lo, hi);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint4 t = part[0];
+      for (int w = 1; w < NW; ++w) {
+        t.x += part[w].x;
+        t.y = max(t.y, part[w].y);
+        t.z = min(t.z, part[w].z);
+        t.w = max(t.w, part[w].w);
+      }
+      products[r] = t.x;
+      max_b[r] = t.y;
+      avg_b[r] = t.x / (e - s);
+      left[r] = t.z;
+      right[r] = t.w;
+      if (max_products) atomicMax(max_products, t.x);
+    }
+    __syncthreads();  // part[] is reused by the next row
+  }
+}
+
+size_t analysis_temp_bytes(uint64_t a_rows, uint64_t a_nnz, uint64_t b_rows) {
+  const uint64_t nlong = std::min<uint64_t>(a_rows, a_nnz / (kLongAIters * 4) + 1);
+  return (b_rows + 1) * 16 + 256 + nlong * 4;
+}
+
 int launch_analysis(const DevCsr &A, const DevCsr &B, const RowInfo &ri,
-                    uint32_t *max_a_nnz, uint32_t *max_products,
+                    uint32_t *max_a_nnz, uint32_t *max_products, void *temp,
                     hipStream_t s) {
   if (A.rows == 0) return OC_OK;
+  OC_REQUIRE(A.rows < (1ull << 32) && B.rows < (1ull << 32),
+             "analysis: more than 2^32 - 1 rows");
+  void *own = nullptr;
+  if (!temp) {
+    HIP_TRY(hipMallocAsync(&own, analysis_temp_bytes(A.rows, A.nnz, B.rows), s));
+    temp = own;
+  }
+  uint4 *info = (uint4 *)temp;
+  uint32_t *long_count = (uint32_t *)(info + B.rows + 1);
+  uint32_t *long_rows = long_count + 64;
+  HIP_TRY(hipMemsetAsync(long_count, 0, 4, s));
   if (max_a_nnz) HIP_TRY(hipMemsetAsync(max_a_nnz, 0, 4, s));
   if (max_products) HIP_TRY(hipMemsetAsync(max_products, 0, 4, s));
+  if (B.rows)
+    hipLaunchKernelGGL(k_brow_info, dim3(std::min<uint64_t>(cdiv(B.rows, 256), 1u << 20)),
+                       dim3(256), 0, s, B.rp, B.ci, (uint32_t)B.rows, info);
   const double avg = (double)A.nnz / (double)A.rows;
   const uint32_t rows = (uint32_t)A.rows;
+  // grid-stride over at most 64 workgroups per CU
+  const uint32_t max_grid = 64 * std::max<uint32_t>(tables().num_cu, 1);
 #define GO(G)                                                                 \
   hipLaunchKernelGGL(k_analysis<G>,                                           \
-                     dim3(std::min<uint32_t>(cdiv(rows, 256 / G), 1u << 20)), \
-                     dim3(256), 0, s, A.rp, A.ci, rows, B.rp, B.ci,           \
+                     dim3(std::min<uint32_t>(cdiv(rows, 256 / G), max_grid)), \
+                     dim3(256), 0, s, A.rp, A.ci, rows, info,                 \
                      ri.products, ri.max_b, ri.avg_b, ri.left, ri.right,      \
-                     max_a_nnz, max_products)
+                     max_a_nnz, max_products, long_rows, long_count)
   if (avg <= 4) GO(4);
   else if (avg <= 8) GO(8);
   else if (avg <= 16) GO(16);
   else if (avg <= 32) GO(32);
   else GO(64);
 #undef GO
+  // a row is long only past kLongAIters * 4 entries, so the list holds at
+  // most nnz / (kLongAIters * 4) rows, the size analysis_temp_bytes gives it
+  hipLaunchKernelGGL(k_analysis_long, dim3(1024), dim3(kLongBlock), 0, s, A.rp, A.ci, info,
+                     ri.products, ri.max_b, ri.avg_b, ri.left, ri.right, max_products,
+                     long_rows, long_count);
   LAUNCH_CHECK();
+  if (own) HIP_TRY(hipFreeAsync(own, s));
   return OC_OK;
 }
 
@@ -784,16 +912,34 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_copy_rows(
     uint64_t nrows, const uint32_t *__restrict__ rows,
     const uint32_t *__restrict__ nnz, const uint64_t *__restrict__ src_off,
-    const uint32_t *__restrict__ sc, const T *__restrict__ sv,
-    const uint64_t *__restrict__ dst_off, uint32_t *__restrict__ dc,
-    T *__restrict__ dv) {
+    const uint32_t *__restrict__ cap, const uint32_t *__restrict__ sc,
+    const T *__restrict__ sv, const uint64_t *__restrict__ dst_off,
+    uint32_t *__restrict__ dc, T *__restrict__ dv) {
   const int lane = lane_id();
   for (uint64_t i = blockIdx.x * 4ull + threadIdx.x / kWave; i < nrows;
        i += gridDim.x * 4ull) {
     const uint32_t r = rows ? rows[i] : (uint32_t)i;
     const uint32_t n = nnz[r];
+    if (cap && n > cap[r]) continue;  // outgrew its slot: computed in place
     const uint64_t so = src_off[r], d = dst_off[r];
-    for (uint32_t j = lane; j < n; j += kWave) {
+    // eight loads in flight per lane before the stores: a long row is
+    // copied by one wave, so its copy time is latency x n / (64 x 8)
+    uint32_t j = lane;
+    for (; j + 7 * kWave < n; j += 8 * kWave) {
+      uint32_t c[8];
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        c[u] = sc[so + j + u * kWave];
+        v[u] = sv[so + j + u * kWave];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        dc[d + j + u * kWave] = c[u];
+        dv[d + j + u * kWave] = v[u];
+      }
+    }
+    for (; j < n; j += kWave) {
       dc[d + j] = sc[so + j];
       dv[d + j] = sv[so + j];
     }
@@ -801,7 +947,7 @@ __global__ __launch_bounds__(256) void k_copy_rows(
 }
 
 int launch_copy_rows(int vtype, uint64_t nrows, const uint32_t *rows,
-                     const uint32_t *nnz, const uint64_t *src_off,
+                     const uint32_t *nnz, const uint64_t *src_off, const uint32_t *cap,
                      const uint32_t *src_cols, const void *src_vals,
                      const uint64_t *dst_off, uint32_t *dst_cols,
                      void *dst_vals, hipStream_t s) {
@@ -809,11 +955,11 @@ int launch_copy_rows(int vtype, uint64_t nrows, const uint32_t *rows,
   dim3 grid(std::min<uint32_t>(cdiv(nrows, 4), 1u << 20));
   if (vtype == OC_F64)
     hipLaunchKernelGGL(k_copy_rows<double>, grid, dim3(256), 0, s, nrows, rows,
-                       nnz, src_off, src_cols, (const double *)src_vals, dst_off,
+                       nnz, src_off, cap, src_cols, (const double *)src_vals, dst_off,
                        dst_cols, (double *)dst_vals);
   else
     hipLaunchKernelGGL(k_copy_rows<float>, grid, dim3(256), 0, s, nrows, rows,
-                       nnz, src_off, src_cols, (const float *)src_vals, dst_off,
+                       nnz, src_off, cap, src_cols, (const float *)src_vals, dst_off,
                        dst_cols, (float *)dst_vals);
   LAUNCH_CHECK();
   return OC_OK;

@@ -47,8 +47,11 @@ int launch_mlp_features(const DevCsr &A, const DevCsr &B, const RowInfo &ri, flo
 int launch_mlp_estimates(const oc_mlp *m, const float *feat, uint64_t n, const RowInfo &ri,
                          double q, uint32_t *est, hipStream_t s);
 
+// temp: analysis_temp_bytes(A.rows, A.nnz, B.rows) bytes, or null to have
+// the launcher allocate (stream-ordered) and free it
+size_t analysis_temp_bytes(uint64_t a_rows, uint64_t a_nnz, uint64_t b_rows);
 int launch_analysis(const DevCsr &A, const DevCsr &B, const RowInfo &ri,
-                    uint32_t *max_a_nnz, uint32_t *max_products,
+                    uint32_t *max_a_nnz, uint32_t *max_products, void *temp,
                     hipStream_t s);
 size_t scan_temp_bytes(uint64_t n);
 int launch_scan_u32(const uint32_t *in, uint64_t *out, uint64_t n, void *temp,
@@ -86,8 +89,9 @@ int launch_binning(const BinParams &bp, uint64_t rows, uint8_t *bin_of_row,
                    hipStream_t s);
 
 // Copy rows (all rows, or those listed) from a temp layout into C.
+// A row whose count exceeds cap[row] (when cap is given) is skipped.
 int launch_copy_rows(int vtype, uint64_t nrows, const uint32_t *rows,
-                     const uint32_t *nnz, const uint64_t *src_off,
+                     const uint32_t *nnz, const uint64_t *src_off, const uint32_t *cap,
                      const uint32_t *src_cols, const void *src_vals,
                      const uint64_t *dst_off, uint32_t *dst_cols,
                      void *dst_vals, hipStream_t s);

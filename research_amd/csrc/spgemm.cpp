@@ -40,35 +40,78 @@ int bits_needed(uint64_t x) {
   return b;
 }
 
-// Stream-ordered allocations released when the run ends.
+// Device memory for the temporaries of a handle's runs: chunks from
+// hipMalloc, handed out by a bump pointer and all reset when a run ends
+// (every run ends with a stream synchronisation).  Repeated runs reuse the
+// same memory; handing every temporary back to the stream-ordered pool
+// instead fragments it under products of varying size until allocations
+// stall while the pool trims; and device memory freed and allocated again
+// is cleared by the driver first, at tens of GB/s.  The chunks stay with the
+// handle until it is destroyed, so a repeated product allocates nothing
+// after its first run.
+struct ChunkStore {
+  struct Chunk {
+    uint8_t *base;
+    size_t size, used;
+  };
+  std::vector<Chunk> chunks;
+  size_t total() const {
+    size_t t = 0;
+    for (const Chunk &c : chunks) t += c.size;
+    return t;
+  }
+  void drop() {
+    for (Chunk &c : chunks) (void)hipFree(c.base);
+    chunks.clear();
+  }
+};
+
+// One run's allocations from the handle's chunk store.
 struct Arena {
-  hipStream_t s;
-  std::vector<void *> ptrs;
-  explicit Arena(hipStream_t st) : s(st) {}
+  ChunkStore *cs;
+  struct Rec {
+    size_t chunk, off;
+    void *p;
+  };
+  std::vector<Rec> recs;
+  explicit Arena(ChunkStore *c) : cs(c) {}
   ~Arena() { release(); }
   template <typename T>
   int get(T **p, uint64_t count) {
-    size_t bytes = (size_t)std::max<uint64_t>(count, 1) * sizeof(T);
-    bytes = (bytes + 255) & ~(size_t)255;
-    hipError_t e = hipMallocAsync((void **)p, bytes, s);
+    const size_t bytes = ((size_t)std::max<uint64_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+    for (size_t i = 0; i < cs->chunks.size(); ++i) {
+      ChunkStore::Chunk &c = cs->chunks[i];
+      if (c.size - c.used >= bytes) {
+        *p = (T *)(c.base + c.used);
+        recs.push_back(Rec{i, c.used, (void *)*p});
+        c.used += bytes;
+        return OC_OK;
+      }
+    }
+    uint8_t *base = nullptr;
+    const size_t size = std::max<size_t>(bytes, 64u << 20);
+    hipError_t e = hipMalloc((void **)&base, size);
     if (e != hipSuccess) {
+      (void)hipGetLastError();
       set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
       return OC_ERR_NOMEM;
     }
-    ptrs.push_back((void *)*p);
+    cs->chunks.push_back(ChunkStore::Chunk{base, size, bytes});
+    *p = (T *)base;
+    recs.push_back(Rec{cs->chunks.size() - 1, 0, (void *)base});
     return OC_OK;
   }
+  // Only the latest allocation is given back (stack order); the others
+  // stay until the run ends.
   void free(void *p) {
-    for (size_t i = 0; i < ptrs.size(); ++i)
-      if (ptrs[i] == p) {
-        (void)hipFreeAsync(p, s);
-        ptrs.erase(ptrs.begin() + (long)i);
-        return;
-      }
+    if (!recs.empty() && recs.back().p == p) {
+      cs->chunks[recs.back().chunk].used = recs.back().off;
+      recs.pop_back();
+    }
   }
   void release() {
-    for (void *p : ptrs) (void)hipFreeAsync(p, s);
-    ptrs.clear();
+    recs.clear();
+    for (ChunkStore::Chunk &c : cs->chunks) c.used = 0;
   }
 };
 
@@ -135,9 +178,9 @@ struct oc_spgemm {
   uint32_t bin1[kBinSlots + 1] = {0}, bin2[kBinSlots + 1] = {0};
   bool have_bin1 = false, have_bin2 = false;
   uint64_t total_est = 0;
+  uint64_t est_spare_need = 0;       // entries the overflow rows reserved
+  bool est_separate_spare = false;   // ... in a buffer of their own
   uint32_t overflow_rows = 0, sym_overflow_rows = 0, precise_recount = 0;
-  bool est_fallback = false;  // estimation workflow fell back to precise
-  uint64_t est_spare_need = 0;
   int vtype = OC_F64;
   double total_ms = 0;
   std::map<std::string, double> stage_ms;
@@ -154,7 +197,9 @@ struct oc_spgemm {
   const oc_mlp *model = nullptr;
   oc_mlp *owned_model = nullptr;
   double model_q = 0.99;
-  bool learned_run = false;
+  bool learned_run = false;        // estimation workflow on learned capacities
+  bool learned_consulted = false;  // the predictor ran and picked the workflow
+  uint64_t learned_total = 0;      // sum of the learned capacities
   // Numeric plan of a precise run kept for value-only replays: the row
   // information, numeric bins and scratch stay allocated; C is the caller's.
   struct Plan {
@@ -184,7 +229,10 @@ struct oc_spgemm {
     for (void *p : plan.mem) (void)hipFree(p);
     plan = Plan();
   }
+  ChunkStore mem;  // temporaries of the runs (see ChunkStore)
   ~oc_spgemm() {
+    if (!mem.chunks.empty()) (void)hipDeviceSynchronize();
+    mem.drop();
     drop_plan();
     if (own_stream) (void)hipStreamDestroy(own_stream);
     oc_mlp_destroy(owned_model);
@@ -291,6 +339,7 @@ struct Run {
   uint32_t *scal = nullptr;
   void *scan_tmp = nullptr;
   uint8_t *hll_b = nullptr;
+  uint32_t *learned_est = nullptr;  // learned capacities (learned_capacities)
   void *win_scratch = nullptr;
   uint32_t *tile_ts = nullptr;
   uint32_t tile_logw = 0, tile_t1 = 0;
@@ -315,7 +364,7 @@ struct Run {
     if (c.track_stage_times && n && !replay) fams.push_back(Fam{name, stage, numeric, rows, n});
   }
   Run(oc_spgemm *hh, hipStream_t st, oc_csr *cc)
-      : h(hh), c(hh->cfg), s(st), ar(st), mk(st), C(cc) {}
+      : h(hh), c(hh->cfg), s(st), ar(&hh->mem), mk(st), C(cc) {}
 
   AccArgs acc() const {
     AccArgs a;
@@ -493,6 +542,7 @@ struct Run {
 
   int prologue_and_analysis();
   int sampling();
+  int learned_capacities(bool *use_est);
   int run_usparse();
   int run_precise();
   int run_est();
@@ -522,7 +572,10 @@ int Run::prologue_and_analysis() {
                          !c.use_hybrid_accumulator) &&
                        B.cols > (uint64_t)c.numeric_hybrid_hash_kernel_row_length_threshold;
   mk.mark("prologue");
-  OC_TRY(launch_analysis(A, B, ri, scal, scal + 1, s));
+  uint8_t *atmp;
+  OC_TRY(ar.get(&atmp, analysis_temp_bytes(m, A.nnz, B.rows)));
+  OC_TRY(launch_analysis(A, B, ri, scal, scal + 1, atmp, s));
+  ar.free(atmp);  // stream-ordered: released once the analysis has run
   mk.mark("analysis_product_calc");
   OC_TRY(launch_scan_u32(ri.products, psum_prod, m, scan_tmp, s));
   mk.mark("analysis_reduce");
@@ -563,6 +616,31 @@ int Run::sampling() {
   decide_sampling(c, *h, hp, he, d);
   d.hash_largest = hl;
   mk.mark("sampling_var_calc_host");
+  return OC_OK;
+}
+
+// Learned capacities of every row, and the workflow they pick.  The
+// estimation workflow saves the symbolic pass (a second sweep over all
+// products) but adds a copy of C from the estimated slots to its final
+// place; it pays when the products outnumber the predicted output by at
+// least learned_min_compression.
+int Run::learned_capacities(bool *use_est) {
+  const uint64_t m = A.rows;
+  OC_TRY(ar.get(&learned_est, m + 1));
+  float *feat;
+  OC_TRY(ar.get(&feat, m * OC_MLP_FEATURES));
+  OC_TRY(launch_mlp_features(A, B, ri, feat, s));
+  OC_TRY(launch_mlp_estimates(h->model, feat, m, ri, h->model_q, learned_est, s));
+  ar.free(feat);
+  uint64_t *ps;
+  OC_TRY(ar.get(&ps, m + 1));
+  OC_TRY(launch_scan_u32(learned_est, ps, m, scan_tmp, s));
+  OC_TRY(readback(&h->learned_total, ps + m, 1, s));
+  ar.free(ps);
+  h->learned_consulted = true;
+  *use_est = (double)h->total_products >=
+             c.learned_min_compression * (double)std::max<uint64_t>(h->learned_total, 1);
+  mk.mark("est_hll_merge");
   return OC_OK;
 }
 
@@ -617,7 +695,7 @@ int Run::run_usparse() {
     mk.mark("numeric_usparse_outlier");
   }
   OC_TRY(alloc_c_from_counts(nnz));
-  OC_TRY(launch_copy_rows(A.vtype, m, nullptr, nnz, psum_prod, tc, tv,
+  OC_TRY(launch_copy_rows(A.vtype, m, nullptr, nnz, psum_prod, nullptr, tc, tv,
                           (const uint64_t *)C->row_offsets, C->col_ids, C->values, s));
   mk.mark("epilogue_copy");
   return OC_OK;
@@ -762,10 +840,7 @@ int Run::run_est() {
   if (h->est_override) {
     HIP_TRY(hipMemcpyAsync(est, h->est_override, m * 4, hipMemcpyDeviceToDevice, s));
   } else if (h->learned_run) {
-    float *feat;
-    OC_TRY(ar.get(&feat, m * OC_MLP_FEATURES));
-    OC_TRY(launch_mlp_features(A, B, ri, feat, s));
-    OC_TRY(launch_mlp_estimates(h->model, feat, m, ri, h->model_q, est, s));
+    est = learned_est;  // computed when the predictor picked this workflow
   } else {
     uint8_t *tmp;
     OC_TRY(ar.get(&tmp, hll_merge_temp_bytes(m)));
@@ -801,6 +876,8 @@ int Run::run_est() {
   OC_TRY(launch_scan_u32(est, psum_est, m, scan_tmp, s));
   OC_TRY(readback(&h->total_est, psum_est + m, 1, s));
   mk.mark("numeric_binning");
+  // estimated slots plus a spare region for the rows that outgrow theirs
+  // (SpGEMM.cuh:1560-1587)
   const double coe = c.hll_numeric_malloc_expansion_coe;
   const uint64_t cap_total = std::max<uint64_t>((uint64_t)((double)h->total_est * coe), h->total_est);
   uint32_t *tc;
@@ -841,6 +918,9 @@ int Run::run_est() {
   uint64_t *src_off;
   OC_TRY(ar.get(&src_off, m + 1));
   HIP_TRY(hipMemcpyAsync(src_off, psum_est, (m + 1) * 8, hipMemcpyDeviceToDevice, s));
+  bool separate = false;  // overflow rows in a buffer of their own
+  uint32_t *sc = nullptr;
+  uint8_t *sv = nullptr;
   if (novf) {
     unsigned long long *used;
     OC_TRY(ar.get(&used, 1));
@@ -849,28 +929,42 @@ int Run::run_est() {
                                h->total_est, used, src_off, s));
     unsigned long long hu = 0;
     OC_TRY(readback(&hu, used, 1, s));
-    if (h->total_est + hu > cap_total) {
-      // The spare region (SpGEMM.cuh:1583-1587 raises here) cannot hold the
-      // rows the estimates missed: recompute this product with exact counts.
-      h->est_fallback = true;
-      h->est_spare_need = hu;
-      ar.free(tc);
-      ar.free(tv);
-      h->d.ana2_type = 0;
-      mk.mark("numeric_overflow");
-      return run_precise();
-    }
+    h->est_spare_need = hu;
     NumOut o2 = o;
-    o2.off = src_off;
     o2.cap = nullptr;
     o2.nnz_hint = nullptr;  // the estimate was too small for these rows
+    o2.off = src_off;
+    if (h->total_est + hu > cap_total) {
+      // The spare region behind the estimated slots (SpGEMM.cuh:1583-1587
+      // raises here) cannot hold the rows the estimates missed: they get a
+      // buffer of exactly the room they reserve, offsets from 0.
+      separate = true;
+      HIP_TRY(hipMemsetAsync(used, 0, 8, s));
+      OC_TRY(launch_assign_spare(ovf_rows, novf, ri.products, ri.left, ri.right, 0, used,
+                                 src_off, s));
+      OC_TRY(ar.get(&sc, hu));
+      OC_TRY(ar.get(&sv, hu * vsize(A.vtype)));
+      o2.cols = sc;
+      o2.vals = sv;
+    }
     OC_TRY(big_rows(ovf_rows, novf, o2));
   }
+  h->est_separate_spare = separate;
   fam("num_overflow", "numeric_overflow", true, ovf_rows, novf);
   mk.mark("numeric_overflow");
   OC_TRY(alloc_c_from_counts(nnz));
-  OC_TRY(launch_copy_rows(A.vtype, m, nullptr, nnz, src_off, tc, tv,
-                          (const uint64_t *)C->row_offsets, C->col_ids, C->values, s));
+  if (!separate) {
+    OC_TRY(launch_copy_rows(A.vtype, m, nullptr, nnz, src_off, nullptr, tc, tv,
+                            (const uint64_t *)C->row_offsets, C->col_ids, C->values, s));
+  } else {
+    // rows within their estimate from the estimated slots (a row that
+    // outgrew its slot is skipped; listed rows that fit are overwritten
+    // next), then the listed rows from their own buffer
+    OC_TRY(launch_copy_rows(A.vtype, m, nullptr, nnz, psum_est, est, tc, tv,
+                            (const uint64_t *)C->row_offsets, C->col_ids, C->values, s));
+    OC_TRY(launch_copy_rows(A.vtype, novf, ovf_rows, nnz, src_off, nullptr, sc, sv,
+                            (const uint64_t *)C->row_offsets, C->col_ids, C->values, s));
+  }
   mk.mark("epilogue_copy");
   return OC_OK;
 }
@@ -963,11 +1057,10 @@ std::string stats_json(const oc_spgemm *h) {
   }
   if (workflow == 3) j.u64("fat_runs_precise_kernels", 1);
   j.boolean("learned_estimates", h->learned_run);
-  if (h->est_fallback) {
-    j.boolean("estimation_fell_back_to_precise", true);
-    j.u64("estimation_overflow_rows", h->overflow_rows);
-    j.u64("estimation_spare_needed", h->est_spare_need);
-    j.u64("estimation_total_est", h->total_est);
+  if (h->learned_consulted) {
+    j.u64("learned_total_capacity", h->learned_total);
+    j.num("learned_products_per_capacity",
+          (double)h->total_products / (double)std::max<uint64_t>(h->learned_total, 1));
   }
   j.close();
   if (workflow <= 1) {
@@ -987,6 +1080,8 @@ std::string stats_json(const oc_spgemm *h) {
     j.u64("overflow_rows", h->overflow_rows);
     j.u64("log2_precision", d.log_p);
     j.u64("total_est", h->total_est);
+    j.u64("spare_needed", h->est_spare_need);
+    j.boolean("spare_separate", h->est_separate_spare);
     j.num("expansion", h->cfg.hll_numeric_malloc_expansion_coe);
     j.close();
   }
@@ -1123,8 +1218,10 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
   h->have_bin1 = h->have_bin2 = false;
   h->total_est = 0;
   h->overflow_rows = h->sym_overflow_rows = h->precise_recount = 0;
-  h->est_fallback = false;
+  h->learned_run = h->learned_consulted = false;
   h->est_spare_need = 0;
+  h->est_separate_spare = false;
+  h->learned_total = 0;
   h->stage_ms.clear();
   std::memset(dC, 0, sizeof(*dC));
   Run r(h, s, dC);
@@ -1149,15 +1246,21 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
     h->d = Decision();
   } else {
     rc = r.prologue_and_analysis();
-    h->learned_run = false;
     if (rc == OC_OK && h->keep_plan) {
       h->d.ana1_type = 1;  // reported as the precise workflow
       h->d.ana2_type = 0;
       rc = r.run_precise();
     } else if (rc == OC_OK) {
-      h->learned_run = h->model != nullptr && h->d.ana1_type == 1 && h->force_workflow != 0;
-      if (h->learned_run) h->d.ana2_type = 1;  // predicted capacities
-      else if (h->d.ana1_type == 1) rc = r.sampling();
+      const bool learned = h->model != nullptr && h->d.ana1_type == 1 && h->force_workflow != 0;
+      bool use_est = false;
+      if (learned) rc = r.learned_capacities(&use_est);
+      if (rc == OC_OK && learned && (use_est || h->force_workflow == 1)) {
+        h->learned_run = true;
+        h->d.ana2_type = 1;  // predicted capacities
+      } else if (rc == OC_OK && h->d.ana1_type == 1) {
+        rc = r.sampling();
+        if (learned) h->d.ana2_type = 0;  // the predictor picked exact counts
+      }
       if (rc == OC_OK && h->force_workflow >= 0 && h->d.ana1_type == 1)
         h->d.ana2_type = h->force_workflow;
       if (rc == OC_OK) {
@@ -1197,8 +1300,11 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
   }
   r.mk.resolve(h->stage_ms, h->total_ms);
   if (rc == OC_OK && h->keep_plan && h->plan.valid) {
-    h->plan.mem = r.ar.ptrs;  // kept for replays, freed by drop_plan
-    r.ar.ptrs.clear();
+    // the plan keeps this run's memory for replays (freed by drop_plan);
+    // the next run starts a new chunk store
+    for (const ChunkStore::Chunk &ck : h->mem.chunks) h->plan.mem.push_back(ck.base);
+    h->mem.chunks.clear();
+    r.ar.recs.clear();
   } else if (h->keep_plan) {
     h->plan.valid = false;
   }
@@ -1241,7 +1347,7 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   int rc = r.numeric_bins(p.brows, p.hoff, p.o, true);
   if (rc == OC_OK) rc = r.band_numeric(p.o);
   h->max_a_nnz = keep_max;
-  if (rc == OC_OK && !r.ar.ptrs.empty()) {
+  if (rc == OC_OK && !r.ar.recs.empty()) {
     set_error("internal: a replay allocated memory");
     rc = OC_ERR_INVALID;
   }

@@ -70,6 +70,7 @@ class _Config(C.Structure):
         ("tile_block", C.c_int32), ("tile_products_cap", C.c_int32),
         ("learned_quantile", C.c_double),
         ("band_log2_cols", C.c_int32), ("band_min_products", C.c_int32),
+        ("learned_min_compression", C.c_double),
     ]
 
 

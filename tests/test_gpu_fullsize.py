@@ -84,13 +84,16 @@ def test_rmat20_full_size(gpu):
     dC.free()
 
 
-@pytest.mark.parametrize("kind", ["exact", "double", "partial", "long_rows"])
+@pytest.mark.parametrize("kind", ["exact", "double", "partial", "long_rows", "all_low"])
 def test_estimation_overflow_forced(gpu, kind):
     """The estimation workflow with host-given per-row estimates: exact
     counts (no row overflows), generous estimates, generous estimates with
     every fifth multi-entry row given an estimate of 1, or three long rows
     (beyond the LDS kernels) given half their count, so those rows overflow
-    into the spare region.  The product must equal the oracle's in every
+    into the spare region, or every multi-entry row given an estimate of 1,
+    whose overflow rows need more room than the spare region sized from the
+    estimates holds (they then get a buffer of their own).  The product must
+    equal the oracle's in every
     case; the overflow count is the rows whose count exceeds the estimate
     after binning raised dense-bin-0 estimates to that bin's width."""
     if kind == "long_rows":
@@ -106,6 +109,8 @@ def test_estimation_overflow_forced(gpu, kind):
     if kind == "partial":
         cand = np.nonzero(exact > 1)[0]
         est[cand[::5]] = 1
+    if kind == "all_low":
+        est[exact > 1] = 1
     if kind == "long_rows":
         bins0, _ = ora.bin_numeric(est, ref["left"], ref["right"], t["hash_num"], t["dense_num"],
                                    1, 1.5, modify_dense_est=1)
@@ -131,6 +136,10 @@ def test_estimation_overflow_forced(gpu, kind):
     assert ovf == want_ovf
     if kind == "partial":
         assert want_ovf > 100
+    if kind == "all_low":
+        assert want_ovf > A.rows // 4 and st["estimation"]["spare_separate"]
+    if kind in ("exact", "double"):
+        assert not st["estimation"]["spare_separate"]
     if kind == "long_rows":
         assert want_ovf == 3 and st["binning_2"][14] - st["binning_2"][13] >= 3
     sp.set_estimates(None)

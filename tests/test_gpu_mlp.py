@@ -135,11 +135,13 @@ def test_training_and_learned_spgemm(gpu):
     dA = A.upload()
     est = m.estimates(dA, dA, 0.99).to_array(np.uint32, A.rows)
     exact, _ = ora.symbolic(A, A)
-    sp = gpu.SpGEMM(gpu.Config(use_ultrasparse_workflow=0))
+    # learned_min_compression 0: the estimation workflow on any capacities
+    sp = gpu.SpGEMM(gpu.Config(use_ultrasparse_workflow=0, learned_min_compression=0.0))
     sp.set_model(m, 0.99)
     dC = sp.run(dA, dA)
     st = sp.get_stats()
     assert st["decision"]["learned_estimates"] and st["decision"]["workflow"] == 1
+    assert st["decision"]["learned_total_capacity"] == int(est.astype(np.uint64).sum())
     assert dC.validate() == 0
     check_product(A, A, dC.download(), 1e-12, "learned rmat13")
     under = int(np.count_nonzero(exact > est))
@@ -147,3 +149,14 @@ def test_training_and_learned_spgemm(gpu):
         f"overflow rows {st['estimation']['overflow_rows']}, "
         f"sum capacity / sum nnz {est.sum() / max(exact.sum(), 1):.2f}")
     assert st["estimation"]["overflow_rows"] <= under + 0
+    # default rule: estimation only when products >= 1.5 x the capacities
+    sp2 = gpu.SpGEMM(gpu.Config(use_ultrasparse_workflow=0))
+    sp2.set_model(m, 0.99)
+    dC2 = sp2.run(dA, dA)
+    st2 = sp2.get_stats()
+    ratio = st2["stats"]["intermediate_product"] / st2["decision"]["learned_total_capacity"]
+    assert abs(ratio - st2["decision"]["learned_products_per_capacity"]) < 1e-9 * ratio
+    assert (st2["decision"]["workflow"] == 1) == (ratio >= 1.5)
+    assert st2["decision"]["learned_estimates"] == (ratio >= 1.5)
+    log(f"learned rule: products / capacity {ratio:.3f} -> workflow {st2['decision']['workflow']}")
+    check_product(A, A, dC2.download(), 1e-12, "learned rmat13, default rule")

@@ -24,8 +24,26 @@ def _chk(gpu, rc):
     gpu._check(rc)
 
 
+def _hubs(ra):
+    """Rows of every length class of the analysis: short rows, rows just
+    below and above the long-row cut of each lane-group width, and one hub
+    row of 40000 entries; a third of the rows (so of B's rows) are empty."""
+    rng = np.random.default_rng(11)
+    n = 50000
+    lens = rng.integers(0, 24, n)
+    lens[rng.random(n) < 0.33] = 0
+    for i, L in enumerate([127, 128, 129, 511, 512, 513, 1024, 1025, 2049, 6000, 40000]):
+        lens[100 + 97 * i] = L
+    ro = np.zeros(n + 1, np.uint32)
+    np.cumsum(lens, out=ro[1:])
+    ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.uint32)
+    return ra.CSR(rows=n, cols=n, row_offsets=ro, col_ids=ci,
+                  values=rng.random(len(ci)))
+
+
 MATS = {
     "laplacian": lambda ra: ra.CSR.laplacian2d(96, 80, seed=0),
+    "hubs": _hubs,
     "rmat13": lambda ra: ra.CSR.rmat(13, 16 << 13, seed=1),
     "er": lambda ra: ra.CSR.erdos_renyi(5000, 30.0, seed=2),
     "rect": lambda ra: ra.CSR.rect_band(3000, 50000, 8.0, 0.3, 4096, 8, vtype=1, seed=3),
