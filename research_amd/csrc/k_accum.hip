@@ -276,6 +276,8 @@ bad:
 }
 
 // ========================================================== symbolic hash
+constexpr uint32_t kSymHeavy = 56;  // listed heavy B rows per row (hdr[8..63])
+
 // Exact row nnz from an LDS hash set (kernels/AccumulatorHash.cuh:11-152).
 // With limit < S (estimated-compaction tables) a row whose occupancy
 // reaches the limit, or whose probe sequence wraps, is listed as overflow
@@ -300,32 +302,59 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
   const bool track = limit < S;
   uint32_t mine = 0;
   bool ovf = false;
+  auto insert = [&](uint32_t c) {
+    if (ovf) return;
+    uint32_t slot = hash_slot(c, S);
+    uint32_t probes = 0;
+    for (; probes < S; ++probes) {
+      const uint32_t v = tab[slot];
+      if (v == c) break;
+      if (v == kEmpty) {
+        const uint32_t old = atomicCAS(&tab[slot], kEmpty, c);
+        if (old == kEmpty) {
+          ++mine;
+          if (track) atomicAdd(&hdr[0], 1u);
+          break;
+        }
+        if (old == c) break;
+      }
+      if (++slot == S) slot = 0;
+    }
+    if (probes == S) ovf = true;
+  };
+  // A B row far longer than a group's share of the row's products (an
+  // R-MAT hub) would keep its group busy long after the others finish: such
+  // rows are listed first and, when there are at most kSymHeavy of them,
+  // swept by the whole workgroup after the groups have done the rest.
+  const uint32_t share = (a.products[r] + ngroups - 1) / ngroups;
+  const uint32_t heavy = max(8u * (uint32_t)G, 2u * share);
+  for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
+    const uint32_t k = a.a_ci[j];
+    if (lig == 0 && a.b_rp[k + 1] - a.b_rp[k] > heavy) {
+      const uint32_t idx = atomicAdd(&hdr[3], 1u);
+      if (idx < kSymHeavy) hdr[8 + idx] = j;
+    }
+  }
+  __syncthreads();
+  const uint32_t nheavy = hdr[3] <= kSymHeavy ? hdr[3] : 0u;
   for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
     if (track && lds_load(&hdr[0]) > limit) {  // past the load limit
       ovf = true;
       break;
     }
     const uint32_t k = a.a_ci[j];
-    seg_cols<8>(a.b_rp[k], a.b_rp[k + 1], lig, G, a.b_ci, [&](uint32_t c) {
-      if (ovf) return;
-      uint32_t slot = hash_slot(c, S);
-      uint32_t probes = 0;
-      for (; probes < S; ++probes) {
-        const uint32_t v = tab[slot];
-        if (v == c) break;
-        if (v == kEmpty) {
-          const uint32_t old = atomicCAS(&tab[slot], kEmpty, c);
-          if (old == kEmpty) {
-            ++mine;
-            if (track) atomicAdd(&hdr[0], 1u);
-            break;
-          }
-          if (old == c) break;
-        }
-        if (++slot == S) slot = 0;
-      }
-      if (probes == S) ovf = true;
-    });
+    const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
+    if (nheavy && be - bs > heavy) continue;  // listed: swept below
+    seg_cols<8>(bs, be, lig, G, a.b_ci, insert);
+  }
+  __syncthreads();
+  for (uint32_t h = 0; h < nheavy && !ovf; ++h) {
+    if (track && lds_load(&hdr[0]) > limit) {
+      ovf = true;
+      break;
+    }
+    const uint32_t k = a.a_ci[hdr[8 + h]];
+    seg_cols<8>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, insert);
   }
   if (ovf) hdr[1] = 1;
   atomicAdd(&hdr[2], mine);
@@ -434,37 +463,62 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
   uint32_t limit = S;
   if (EST) limit = min(L.nmax - 1, o.cap[r]);
   bool ovf = false;
+  auto insert = [&](uint32_t c, double pv) {
+    if (ovf) return;
+    uint32_t slot = hash_slot(c, S);
+    uint32_t probes = 0;
+    for (; probes < S; ++probes) {
+      const uint32_t v = keys[slot];
+      if (v == c) break;
+      if (v == kEmpty) {
+        const uint32_t old = atomicCAS(&keys[slot], kEmpty, c);
+        if (old == kEmpty) {
+          if (EST) atomicAdd(&hdr[0], 1u);
+          break;
+        }
+        if (old == c) break;
+      }
+      if (++slot == S) slot = 0;
+    }
+    if (probes == S) {
+      ovf = true;
+      return;
+    }
+    lds_add_f64(&vals[slot], pv);
+  };
+  // heavy B rows (far beyond a group's share of the products) are listed
+  // and swept by the whole workgroup, as in k_sym_hash
+  const uint32_t share = (a.products[r] + ngroups - 1) / ngroups;
+  const uint32_t heavy = max(8u * (uint32_t)G, 2u * share);
+  for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
+    const uint32_t k = a.a_ci[j];
+    if (lig == 0 && a.b_rp[k + 1] - a.b_rp[k] > heavy) {
+      const uint32_t idx = atomicAdd(&hdr[3], 1u);
+      if (idx < kSymHeavy) hdr[8 + idx] = j;
+    }
+  }
+  __syncthreads();
+  const uint32_t nheavy = hdr[3] <= kSymHeavy ? hdr[3] : 0u;
   for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
     if (EST && lds_load(&hdr[0]) > limit) {  // more keys than the capacity
       ovf = true;
       break;
     }
     const uint32_t k = a.a_ci[j];
-    const double av = (double)a.a_v[j];
-    seg_vals<4>(a.b_rp[k], a.b_rp[k + 1], lig, G, av, a.b_ci, a.b_v,
-                [&](uint32_t c, double pv) {
-      if (ovf) return;
-      uint32_t slot = hash_slot(c, S);
-      uint32_t probes = 0;
-      for (; probes < S; ++probes) {
-        const uint32_t v = keys[slot];
-        if (v == c) break;
-        if (v == kEmpty) {
-          const uint32_t old = atomicCAS(&keys[slot], kEmpty, c);
-          if (old == kEmpty) {
-            if (EST) atomicAdd(&hdr[0], 1u);
-            break;
-          }
-          if (old == c) break;
-        }
-        if (++slot == S) slot = 0;
-      }
-      if (probes == S) {
-        ovf = true;
-        return;
-      }
-      lds_add_f64(&vals[slot], pv);
-    });
+    const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
+    if (nheavy && be - bs > heavy) continue;  // listed: swept below
+    seg_vals<4>(bs, be, lig, G, (double)a.a_v[j], a.b_ci, a.b_v, insert);
+  }
+  __syncthreads();
+  for (uint32_t h = 0; h < nheavy && !ovf; ++h) {
+    if (EST && lds_load(&hdr[0]) > limit) {
+      ovf = true;
+      break;
+    }
+    const uint32_t j = hdr[8 + h];
+    const uint32_t k = a.a_ci[j];
+    seg_vals<4>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, (double)a.a_v[j], a.b_ci,
+                a.b_v, insert);
   }
   if (ovf) hdr[1] = 1;
   __syncthreads();
@@ -919,11 +973,14 @@ __device__ __forceinline__ uint32_t rank_of(uint32_t x, const uint32_t *wpre, co
 // by marking each segment's first position and a workgroup max-scan, gives
 // every position its A entry with one LDS read, and lane-strided positions
 // keep the B gathers coalesced.
+// Entries per B row of the compact tile-start table of the widest tile.
+constexpr uint32_t kWideStarts = 16;
+
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void k_num_rank(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
-    uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1, uint32_t cap, uint32_t pcap,
-    KOut<T> o) {
+    uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1,
+    const uint32_t *__restrict__ tsw, uint32_t cap, uint32_t pcap, KOut<T> o) {
   extern __shared__ __align__(16) uint8_t smem[];
   uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
   uint32_t *sk = reinterpret_cast<uint32_t *>(smem + 256);
@@ -986,8 +1043,12 @@ __global__ __launch_bounds__(BS) void k_num_rank(
     };
     // Tile starts of this thread's A entry.  Single-chunk rows walk their
     // tiles in order, so a tile's end is the next tile's start and the end
-    // after it is loaded one tile ahead.
+    // after it is loaded one tile ahead.  Rows at the widest tile read the
+    // compact table (all of a B row's starts in one 64-byte slot, so the
+    // line a row's tiles share stays cached between them).
+    const bool wide = tsw != nullptr && lw == RankShape<BS>::MAXLOG;
     auto tsv = [&](uint32_t k, uint32_t tt) {
+      if (wide) return tsw[(uint64_t)k * kWideStarts + min(kWideStarts - 1, tt)];
       return ts[(uint64_t)k * T1 + min(T1 - 1, tt << step)];
     };
     uint32_t ts_cur = 0, ts_end = 0;
@@ -1167,9 +1228,23 @@ __global__ __launch_bounds__(BS) void k_num_rank(
   }
 }
 
+uint32_t rank_wide_log(uint32_t bs) {
+  if (bs == 256) return RankShape<256>::MAXLOG;
+  if (bs == 512) return RankShape<512>::MAXLOG;
+  return RankShape<1024>::MAXLOG;
+}
+
+int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row) {
+  const uint32_t lw = rank_wide_log(bs);
+  const uint64_t tiles = b_cols ? ((b_cols - 1) >> lw) + 1 : 1;
+  if (tiles + 1 > kWideStarts) return 0;
+  *entries_per_row = kWideStarts;
+  return 1;
+}
+
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                    const uint32_t *ts, uint32_t T1, uint32_t pcap, uint32_t bs,
-                    const NumOut &o, void *scratch, hipStream_t s) {
+                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, uint32_t pcap,
+                    uint32_t bs, const NumOut &o, void *scratch, hipStream_t s) {
   if (nrows == 0) return OC_OK;
   const Tables &t = tables();
   uint32_t *counter = (uint32_t *)scratch;
@@ -1189,7 +1264,7 @@ int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint
     const size_t lds = fixed + (size_t)cap * 12;                                    \
     OC_TRY(set_lds(k_num_rank<T, B>, lds));                                         \
     hipLaunchKernelGGL((k_num_rank<T, B>), dim3(grid), dim3(B), lds, s, kargs<T>(a), \
-                       rows, nrows, counter, logT, ts, T1, cap, pcap, kout<T>(o));  \
+                       rows, nrows, counter, logT, ts, T1, tsw, cap, pcap, kout<T>(o)); \
   } while (0)
   if (a.A.vtype == OC_F64) {
     if (bs == 256) GO(double, 256);

@@ -163,10 +163,14 @@ int launch_tile_starts(const DevCsr &B, uint32_t logW, uint32_t T1, uint32_t *ts
 int launch_num_tiles(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logW,
                      uint32_t bs, const uint32_t *ts, uint32_t T1, const NumOut &o,
                      void *scratch, hipStream_t s, const uint32_t *nrows_dev = nullptr);
-// Rank-compressed tiles (tile-start table of granularity 2^logT columns).
+// Rank-compressed tiles (tile-start table of granularity 2^logT columns;
+// tsw, when given, the compact table of the widest tile: wide_starts_plan
+// entries per B row, built by launch_tile_starts at rank_wide_log(bs)).
+uint32_t rank_wide_log(uint32_t bs);
+int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row);
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                    const uint32_t *ts, uint32_t T1, uint32_t pcap, uint32_t bs,
-                    const NumOut &o, void *scratch, hipStream_t s);
+                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, uint32_t pcap,
+                    uint32_t bs, const NumOut &o, void *scratch, hipStream_t s);
 // Ultra-sparse expand-sort-compress for all rows with products <= esc_n.
 int launch_esc(int esc_n, const AccArgs &a, const NumOut &o, hipStream_t s);
 

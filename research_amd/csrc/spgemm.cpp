@@ -210,7 +210,7 @@ struct oc_spgemm {
     const uint32_t *brows = nullptr;
     uint32_t hoff[kBinSlots + 1] = {0};
     NumOut o{};
-    uint32_t *tile_ts = nullptr;
+    uint32_t *tile_ts = nullptr, *tile_tsw = nullptr;
     uint32_t tile_logw = 0, tile_t1 = 0, max_a_nnz = 0;
     int tiles_ok = -1;
     void *win_scratch = nullptr;
@@ -341,7 +341,7 @@ struct Run {
   uint8_t *hll_b = nullptr;
   uint32_t *learned_est = nullptr;  // learned capacities (learned_capacities)
   void *win_scratch = nullptr;
-  uint32_t *tile_ts = nullptr;
+  uint32_t *tile_ts = nullptr, *tile_tsw = nullptr;
   uint32_t tile_logw = 0, tile_t1 = 0;
   int tiles_ok = -1;
   // long rows in column bands (precise workflow)
@@ -396,6 +396,12 @@ struct Run {
     if (tiles_ok) {
       OC_TRY(ar.get(&tile_ts, bytes / 4));
       OC_TRY(launch_tile_starts(B, tile_logw, tile_t1, tile_ts, s));
+      size_t per = 0;
+      const uint32_t bs = (uint32_t)h->cfg.tile_block;
+      if (h->cfg.use_tiled_numeric == 2 && wide_starts_plan(B.cols, bs, &per)) {
+        OC_TRY(ar.get(&tile_tsw, B.rows * per));
+        OC_TRY(launch_tile_starts(B, rank_wide_log(bs), (uint32_t)per, tile_tsw, s));
+      }
       mk.mark("numeric_tile_starts");
     }
     return OC_OK;
@@ -406,7 +412,7 @@ struct Run {
     OC_TRY(window_scratch());
     OC_TRY(tiles());
     if (tiles_ok && h->cfg.use_tiled_numeric == 2)
-      return launch_num_rank(acc(), rows, n, tile_logw, tile_ts, tile_t1,
+      return launch_num_rank(acc(), rows, n, tile_logw, tile_ts, tile_t1, tile_tsw,
                              (uint32_t)h->cfg.tile_products_cap,
                              (uint32_t)h->cfg.tile_block, o,
                              win_scratch, s);
@@ -810,6 +816,7 @@ int Run::run_precise() {
     std::memcpy(p.hoff, hoff, sizeof(hoff));
     p.o = o;
     p.tile_ts = tile_ts;
+    p.tile_tsw = tile_tsw;
     p.tile_logw = tile_logw;
     p.tile_t1 = tile_t1;
     p.tiles_ok = tiles_ok;
@@ -1331,6 +1338,7 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   r.B = p.B;
   r.ri = p.ri;
   r.tile_ts = p.tile_ts;
+  r.tile_tsw = p.tile_tsw;
   r.tile_logw = p.tile_logw;
   r.tile_t1 = p.tile_t1;
   r.tiles_ok = p.tiles_ok;
