@@ -275,8 +275,37 @@ bad:
   return OC_ERR_INVALID;
 }
 
+// ============================================================ heavy rows
+// A B row far longer than a lane group's share of the row's products (an
+// R-MAT hub) keeps its group busy long after the others finish.  The
+// groups list such rows first; when there are at most `cap` of them the
+// whole workgroup sweeps each listed row after the groups have done the
+// rest.  Returns the number listed (0: none, or too many to list, and the
+// groups take every row); the listing pass, which ends with a workgroup
+// barrier, is skipped when the row's longest B row is not heavy.
+template <typename KA>
+__device__ __forceinline__ uint32_t list_heavy(const KA &a, uint32_t r, uint32_t a0,
+                                               uint32_t a1, int G, int gid, int ngroups,
+                                               int lig, uint32_t *count, uint32_t *list,
+                                               uint32_t cap, uint32_t *heavy) {
+  const uint32_t share = (a.products[r] + ngroups - 1) / ngroups;
+  *heavy = max(8u * (uint32_t)G, 2u * share);
+  if (a.max_b[r] <= *heavy) return 0u;  // no heavy row: no listing pass
+  for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
+    const uint32_t k = a.a_ci[j];
+    if (lig == 0 && a.b_rp[k + 1] - a.b_rp[k] > *heavy) {
+      const uint32_t idx = atomicAdd(count, 1u);
+      if (idx < cap) list[idx] = j;
+    }
+  }
+  __syncthreads();
+  const uint32_t n = *count;
+  return n <= cap ? n : 0u;
+}
+
 // ========================================================== symbolic hash
 constexpr uint32_t kSymHeavy = 56;  // listed heavy B rows per row (hdr[8..63])
+constexpr uint32_t kDenseHeavy = 32;  // dense kernels: hdr[32..63], count hdr[4]
 
 // Exact row nnz from an LDS hash set (kernels/AccumulatorHash.cuh:11-152).
 // With limit < S (estimated-compaction tables) a row whose occupancy
@@ -322,21 +351,9 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
     }
     if (probes == S) ovf = true;
   };
-  // A B row far longer than a group's share of the row's products (an
-  // R-MAT hub) would keep its group busy long after the others finish: such
-  // rows are listed first and, when there are at most kSymHeavy of them,
-  // swept by the whole workgroup after the groups have done the rest.
-  const uint32_t share = (a.products[r] + ngroups - 1) / ngroups;
-  const uint32_t heavy = max(8u * (uint32_t)G, 2u * share);
-  for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
-    const uint32_t k = a.a_ci[j];
-    if (lig == 0 && a.b_rp[k + 1] - a.b_rp[k] > heavy) {
-      const uint32_t idx = atomicAdd(&hdr[3], 1u);
-      if (idx < kSymHeavy) hdr[8 + idx] = j;
-    }
-  }
-  __syncthreads();
-  const uint32_t nheavy = hdr[3] <= kSymHeavy ? hdr[3] : 0u;
+  uint32_t heavy;
+  const uint32_t nheavy =
+      list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
   for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
     if (track && lds_load(&hdr[0]) > limit) {  // past the load limit
       ovf = true;
@@ -384,16 +401,26 @@ __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
   }
   const uint32_t nwords = ((hi - lo) >> 5) + 1;
   for (uint32_t i = threadIdx.x; i < nwords; i += BS) bits[i] = 0;
-  if (threadIdx.x == 0) hdr[0] = 0;
+  if (threadIdx.x == 0) hdr[0] = hdr[4] = 0;
   __syncthreads();
   const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
   const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
+  auto mark = [&](uint32_t col) {
+    const uint32_t c = col - lo;
+    atomicOr(&bits[c >> 5], 1u << (c & 31));
+  };
+  uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
+  const uint32_t nheavy =
+      list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[4], hdr + 32, kDenseHeavy, &heavy);
   for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
     const uint32_t k = a.a_ci[j];
-    seg_cols<8>(a.b_rp[k], a.b_rp[k + 1], lig, G, a.b_ci, [&](uint32_t col) {
-      const uint32_t c = col - lo;
-      atomicOr(&bits[c >> 5], 1u << (c & 31));
-    });
+    const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
+    if (nheavy && be - bs > heavy) continue;
+    seg_cols<8>(bs, be, lig, G, a.b_ci, mark);
+  }
+  for (uint32_t h = 0; h < nheavy; ++h) {
+    const uint32_t k = a.a_ci[hdr[32 + h]];
+    seg_cols<8>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, mark);
   }
   __syncthreads();
   uint32_t cnt = 0;
@@ -486,19 +513,9 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
     }
     lds_add_f64(&vals[slot], pv);
   };
-  // heavy B rows (far beyond a group's share of the products) are listed
-  // and swept by the whole workgroup, as in k_sym_hash
-  const uint32_t share = (a.products[r] + ngroups - 1) / ngroups;
-  const uint32_t heavy = max(8u * (uint32_t)G, 2u * share);
-  for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
-    const uint32_t k = a.a_ci[j];
-    if (lig == 0 && a.b_rp[k + 1] - a.b_rp[k] > heavy) {
-      const uint32_t idx = atomicAdd(&hdr[3], 1u);
-      if (idx < kSymHeavy) hdr[8 + idx] = j;
-    }
-  }
-  __syncthreads();
-  const uint32_t nheavy = hdr[3] <= kSymHeavy ? hdr[3] : 0u;
+  uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
+  const uint32_t nheavy =
+      list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
   for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
     if (EST && lds_load(&hdr[0]) > limit) {  // more keys than the capacity
       ovf = true;
@@ -578,18 +595,29 @@ __global__ __launch_bounds__(BS) void k_num_dense(KArgs<T> a,
   uint32_t *bits = reinterpret_cast<uint32_t *>(smem + 256 + (size_t)nwords * 256);
   for (uint32_t i = threadIdx.x; i < nwords * 32; i += BS) vals[i] = 0.0;
   for (uint32_t i = threadIdx.x; i < nwords; i += BS) bits[i] = 0;
+  if (threadIdx.x == 0) hdr[4] = 0;
   __syncthreads();
   const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
   const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
+  auto add = [&](uint32_t col, double pv) {
+    const uint32_t c = col - lo;
+    lds_add_f64(&vals[c], pv);
+    atomicOr(&bits[c >> 5], 1u << (c & 31));
+  };
+  uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
+  const uint32_t nheavy =
+      list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[4], hdr + 32, kDenseHeavy, &heavy);
   for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
     const uint32_t k = a.a_ci[j];
-    const double av = (double)a.a_v[j];
-    seg_vals<4>(a.b_rp[k], a.b_rp[k + 1], lig, G, av, a.b_ci, a.b_v,
-                [&](uint32_t col, double pv) {
-      const uint32_t c = col - lo;
-      lds_add_f64(&vals[c], pv);
-      atomicOr(&bits[c >> 5], 1u << (c & 31));
-    });
+    const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
+    if (nheavy && be - bs > heavy) continue;
+    seg_vals<4>(bs, be, lig, G, (double)a.a_v[j], a.b_ci, a.b_v, add);
+  }
+  for (uint32_t h = 0; h < nheavy; ++h) {
+    const uint32_t j = hdr[32 + h];
+    const uint32_t k = a.a_ci[j];
+    seg_vals<4>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, (double)a.a_v[j], a.b_ci,
+                a.b_v, add);
   }
   __syncthreads();
   const uint32_t wpt = (nwords + BS - 1) / BS;
