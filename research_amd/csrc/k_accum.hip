@@ -1146,7 +1146,7 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       // ---- pass 1: mark the output columns
       uint32_t any = 0, Plast = 0;
       bool keep = false;
-      uint32_t kc[8];  // kept columns of a single-round tile
+      uint32_t kc[8];  // kept columns of round 0 (single-chunk rows)
       for (uint32_t c0 = a0; c0 < a1; c0 += RankShape<BS>::A) {
         const uint32_t nc = min(RankShape<BS>::A, a1 - c0);
         if (!one_chunk) stage(c0, nc);
@@ -1154,8 +1154,13 @@ __global__ __launch_bounds__(BS) void k_num_rank(
         bounds(t, nc, P);
         any |= P;
         Plast = P;
-        keep = one_chunk && P <= RankShape<BS>::ROUND;
-        for (uint32_t p0 = 0; p0 < P; p0 += RankShape<BS>::ROUND) {
+        keep = one_chunk;
+        // single-chunk rows take the rounds last to first, so round 0 (the
+        // only full one when there are two) ends pass 1 with its owner map
+        // in LDS and its columns in registers, and pass 2 gathers it once
+        const uint32_t nr = (P + RankShape<BS>::ROUND - 1) / RankShape<BS>::ROUND;
+        for (uint32_t ri = 0; ri < nr; ++ri) {
+          const uint32_t p0 = (one_chunk ? nr - 1 - ri : ri) * RankShape<BS>::ROUND;
           const uint32_t n = min(RankShape<BS>::ROUND, P - p0);
           build_map(p0, n);
           uint32_t q[8], w[8];
@@ -1185,24 +1190,55 @@ __global__ __launch_bounds__(BS) void k_num_rank(
         break;
       }
       // ---- pass 2: sums by rank (in rank ranges when m > cap)
+      // products [p0, p0 + n) of the current chunk: owner map rebuilt,
+      // columns and values gathered, summed into ranks [r0, r1)
+      auto round_sum = [&](uint32_t p0, uint32_t n, uint32_t r0, uint32_t r1) {
+        build_map(p0, n);
+        uint32_t q[8], w[8];
+        locate(p0, n, q, w);
+        uint32_t col[8];
+        double pv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          col[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
+          pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (col[u] != kEmpty) {
+            const uint32_t rk = rank_of((uint32_t)(col[u] - wbase), wpre, bits);
+            if (rk >= r0 && rk < r1) {
+              lds_add_f64(&acc[rk - r0], pv[u]);
+              cmap[rk - r0] = col[u];
+            }
+          }
+      };
       for (uint32_t r0 = 0; r0 < m; r0 += cap) {
         const uint32_t r1 = min(m, r0 + cap);
         if (keep) {
-          // the single round's owner map is still in LDS: gather the values
-          uint32_t q[8], w[8];
-          locate(0, Plast, q, w);
-          double pv[8];
+          const uint32_t n0 = min(RankShape<BS>::ROUND, Plast);
+          if (r0 == 0 || Plast <= RankShape<BS>::ROUND) {
+            // round 0's owner map is still in LDS and its columns in
+            // registers: gather the values only
+            uint32_t q[8], w[8];
+            locate(0, n0, q, w);
+            double pv[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
+            for (int u = 0; u < 8; ++u) pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
 #pragma unroll
-          for (int u = 0; u < 8; ++u)
-            if (kc[u] != kEmpty) {
-              const uint32_t rk = rank_of((uint32_t)(kc[u] - wbase), wpre, bits);
-              if (rk >= r0 && rk < r1) {
-                lds_add_f64(&acc[rk - r0], pv[u]);
-                cmap[rk - r0] = kc[u];
+            for (int u = 0; u < 8; ++u)
+              if (kc[u] != kEmpty) {
+                const uint32_t rk = rank_of((uint32_t)(kc[u] - wbase), wpre, bits);
+                if (rk >= r0 && rk < r1) {
+                  lds_add_f64(&acc[rk - r0], pv[u]);
+                  cmap[rk - r0] = kc[u];
+                }
               }
-            }
+          } else {
+            round_sum(0, n0, r0, r1);  // a later rank range: round 0's map was replaced
+          }
+          for (uint32_t p0 = RankShape<BS>::ROUND; p0 < Plast; p0 += RankShape<BS>::ROUND)
+            round_sum(p0, min(RankShape<BS>::ROUND, Plast - p0), r0, r1);
         } else {
           for (uint32_t c0 = a0; c0 < a1; c0 += RankShape<BS>::A) {
             const uint32_t nc = min(RankShape<BS>::A, a1 - c0);
@@ -1211,28 +1247,8 @@ __global__ __launch_bounds__(BS) void k_num_rank(
               stage(c0, nc);
               bounds(t, nc, P);
             }
-            for (uint32_t p0 = 0; p0 < P; p0 += RankShape<BS>::ROUND) {
-              const uint32_t n = min(RankShape<BS>::ROUND, P - p0);
-              build_map(p0, n);
-              uint32_t q[8], w[8];
-              locate(p0, n, q, w);
-              uint32_t col[8];
-              double pv[8];
-#pragma unroll
-              for (int u = 0; u < 8; ++u) {
-                col[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
-                pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
-              }
-#pragma unroll
-              for (int u = 0; u < 8; ++u)
-                if (col[u] != kEmpty) {
-                  const uint32_t rk = rank_of((uint32_t)(col[u] - wbase), wpre, bits);
-                  if (rk >= r0 && rk < r1) {
-                    lds_add_f64(&acc[rk - r0], pv[u]);
-                    cmap[rk - r0] = col[u];
-                  }
-                }
-            }
+            for (uint32_t p0 = 0; p0 < P; p0 += RankShape<BS>::ROUND)
+              round_sum(p0, min(RankShape<BS>::ROUND, P - p0), r0, r1);
             if (!one_chunk) __syncthreads();
           }
         }
