@@ -75,6 +75,17 @@ int oc_csr_download(const oc_csr *dev, oc_csr *host, void *stream);
 int oc_csr_load(const char *path, int vtype_expected, oc_csr *host);
 int oc_csr_store(const char *path, const oc_csr *host);
 
+/* MatrixMarket coordinate file -> host CSR (value type vtype, columns
+ * ascending in each row).  Replaces: loadMTX + convert(CSR&, const COO&)
+ * (utils/convert.cpp:83-232): the same accepted headers (real / integer /
+ * double / pattern / complex's real part; general / symmetric / Hermitian,
+ * off-diagonal entries mirrored), skipped comment and blank lines, the same
+ * index checks and messages.  Entries with the same (row, column) keep file
+ * order, or are summed when merge_duplicates is nonzero (no counterpart:
+ * the reference keeps them in an unspecified order).  A 0 index or more
+ * entries than the size line declares is an error. */
+int oc_mtx_load(const char *path, int vtype, int merge_duplicates, oc_csr *host);
+
 /* Correctness check of two host matrices.
  * Replaces: compare (src/CSR.cpp:252-335): equal dims and nnz, equal
  * row_offsets[0..rows), equal col_ids, and value relative error

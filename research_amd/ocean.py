@@ -102,6 +102,7 @@ def _load():
         "oc_csr_download": (I, [P, P, P]),
         "oc_csr_load": (I, [C.c_char_p, I, P]),
         "oc_csr_store": (I, [C.c_char_p, P]),
+        "oc_mtx_load": (I, [C.c_char_p, I, I, P]),
         "oc_csr_compare": (I, [P, P, D, D, P]),
         "oc_config_default": (None, [P]),
         "oc_config_from_json": (I, [P, C.c_char_p]),
@@ -285,6 +286,13 @@ class CSR:
     def load(path: str, vtype: int = -1) -> "CSR":
         m = _Csr()
         _check(lib.oc_csr_load(path.encode(), vtype, C.byref(m)))
+        return CSR._from_oc(m)
+
+    # MatrixMarket coordinate file (utils/convert.cpp:83-232)
+    @staticmethod
+    def load_mtx(path: str, vtype: int = OC_F64, merge_duplicates: bool = False) -> "CSR":
+        m = _Csr()
+        _check(lib.oc_mtx_load(path.encode(), vtype, int(merge_duplicates), C.byref(m)))
         return CSR._from_oc(m)
 
     def store(self, path: str):
