@@ -195,4 +195,16 @@ int launch_band_numeric(const AccArgs &a, const uint32_t *rows, uint32_t nrows, 
                         uint32_t nbands, const uint32_t *ts, const uint64_t *pref,
                         const NumOut &o, uint32_t *counter, hipStream_t s);
 
+// FAT products (k_fat.hip): dense rows over all of B's columns, each A row
+// split over `splits` workgroups; dense [rows][B.cols] FP64 and bits
+// [rows][(B.cols+31)/32] zeroed by the caller.
+size_t fat_lds_bytes(uint64_t b_cols);
+int launch_fat_accumulate(const DevCsr &A, const DevCsr &B, uint32_t splits, int G,
+                          double *dense, uint32_t *bits, hipStream_t s);
+int launch_fat_count(const uint32_t *bits, uint64_t rows, uint64_t b_cols, uint32_t *nnz,
+                     hipStream_t s);
+int launch_fat_compact(int vtype, const double *dense, const uint32_t *bits, uint64_t rows,
+                       uint64_t b_cols, const uint64_t *c_off, uint32_t *c_cols, void *c_vals,
+                       hipStream_t s);
+
 }  // namespace oc

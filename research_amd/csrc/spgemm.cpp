@@ -549,6 +549,7 @@ struct Run {
   int prologue_and_analysis();
   int sampling();
   int learned_capacities(bool *use_est);
+  int run_fat();
   int run_usparse();
   int run_precise();
   int run_est();
@@ -647,6 +648,38 @@ int Run::learned_capacities(bool *use_est) {
   *use_est = (double)h->total_products >=
              c.learned_min_compression * (double)std::max<uint64_t>(h->learned_total, 1);
   mk.mark("est_hll_merge");
+  return OC_OK;
+}
+
+// FAT workflow (SpGEMM.cuh:925-1050): one dense pass over all of B's
+// columns per A row, split over several workgroups per row (k_fat.hip).
+int Run::run_fat() {
+  const uint64_t m = A.rows;
+  const uint64_t nwords = (B.cols + 31) / 32;
+  double *dense;
+  uint32_t *bits, *nnz;
+  OC_TRY(ar.get(&dense, m * B.cols));
+  OC_TRY(ar.get(&bits, m * nwords));
+  OC_TRY(ar.get(&nnz, m + 1));
+  HIP_TRY(hipMemsetAsync(dense, 0, m * B.cols * 8, s));
+  HIP_TRY(hipMemsetAsync(bits, 0, m * nwords * 4, s));
+  mk.mark("numeric_malloc");
+  // about four workgroups per CU in all, each with at least 256 A entries
+  const uint64_t ncu = std::max<uint32_t>(1, tables().num_cu);
+  uint64_t splits = (4 * ncu + m - 1) / m;
+  splits = std::max<uint64_t>(1, std::min<uint64_t>(splits, h->max_a_nnz / 256));
+  // lanes per A entry from B's mean row length (SpGEMM.cuh:963-967)
+  const double avg = B.rows ? (double)B.nnz / (double)B.rows : 0.0;
+  int G = 1;
+  while (G < 64 && (double)G < avg) G <<= 1;
+  OC_TRY(launch_fat_accumulate(A, B, (uint32_t)splits, G, dense, bits, s));
+  OC_TRY(launch_fat_count(bits, m, B.cols, nnz, s));
+  fam("fat", "numeric_fat", true, nullptr, (uint32_t)m);
+  mk.mark("numeric_fat");
+  OC_TRY(alloc_c_from_counts(nnz));
+  OC_TRY(launch_fat_compact(A.vtype, dense, bits, m, B.cols, (const uint64_t *)C->row_offsets,
+                            C->col_ids, C->values, s));
+  mk.mark("epilogue_copy");
   return OC_OK;
 }
 
@@ -1062,7 +1095,6 @@ std::string stats_json(const oc_spgemm *h) {
     j.boolean("hash_largest", d.hash_largest);
     j.boolean("dense_bitmap_query", d.dense_query_bitmap);
   }
-  if (workflow == 3) j.u64("fat_runs_precise_kernels", 1);
   j.boolean("learned_estimates", h->learned_run);
   if (h->learned_consulted) {
     j.u64("learned_total_capacity", h->learned_total);
@@ -1131,7 +1163,7 @@ std::string stats_json(const oc_spgemm *h) {
     j.num("dense", stage(h, "numeric_dense"));
     j.num("window", stage(h, "numeric_window"));
     j.num("overflow", stage(h, "numeric_overflow"));
-    j.num("fat", 0.0);
+    j.num("fat", stage(h, "numeric_fat"));
     j.close();
     j.open("epilogue");
     j.num("scan", stage(h, "epilogue_scan"));
@@ -1273,7 +1305,8 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
       if (rc == OC_OK) {
         const Decision &d = h->d;
         if (d.ana1_type == 0) rc = r.run_usparse();
-        else if (d.ana1_type == 2 || d.ana2_type == 0) rc = r.run_precise();
+        else if (d.ana1_type == 2) rc = r.run_fat();
+        else if (d.ana2_type == 0) rc = r.run_precise();
         else rc = r.run_est();
       }
     }

@@ -51,6 +51,45 @@ def test_rmat_forced_workflows(gpu, workflow):
     log(f"forced {workflow}: err {worst:.2e}, overflow rows {st.get('estimation', {}).get('overflow_rows')}")
 
 
+def _drop_rows(gpu, M, every):
+    """M with every `every`-th row emptied."""
+    ro = M.row_offsets.astype(np.int64)
+    keep = np.ones(M.nnz, bool)
+    lens = np.diff(ro)
+    for r in range(0, M.rows, every):
+        keep[ro[r]:ro[r + 1]] = False
+        lens[r] = 0
+    rp = np.zeros(M.rows + 1, np.uint32)
+    np.cumsum(lens, out=rp[1:])
+    return gpu.CSR(M.rows, M.cols, rp, M.col_ids[keep], M.values[keep])
+
+
+@pytest.mark.parametrize("rows,bcols,vt,tol", [(40, 700, 0, TOL64), (3, 2900, 1, TOL32),
+                                               (300, 64, 0, TOL64)])
+def test_fat_workflow(gpu, rows, bcols, vt, tol):
+    """Short-and-wide A times tall-and-narrow B (A.cols / fat_ratio > A.rows,
+    B.rows / fat_ratio > B.cols, B.cols below the widest dense bin): the
+    FAT workflow (reported as 3), checked against the oracle and against
+    the same product with use_fat_workflow off.  A has empty rows and
+    B empty rows and never-hit columns."""
+    inner = 300000
+    log(f"case: fat {rows} x {inner} times {inner} x {bcols}, vtype {vt}")
+    A = gpu.CSR.rect_band(rows, inner, 1500.0, 0.2, 4096, inner // rows, vtype=vt, seed=5)
+    if rows > 3:
+        A = _drop_rows(gpu, A, 7)
+    Bt = gpu.CSR.rect_band(bcols, inner, 2.5 * inner / bcols, 0.5, 2048, inner // bcols,
+                           vtype=vt, seed=6)
+    B = Bt.transpose()
+    C, st = _run(gpu, A, B, gpu.Config(track_stage_times=1))
+    assert st["decision"]["workflow"] == 3
+    worst, plain = check_product(A, B, C, tol, "fat")
+    C2, st2 = _run(gpu, A, B, gpu.Config(use_fat_workflow=0))
+    assert st2["decision"]["workflow"] != 3
+    assert np.array_equal(C.row_offsets, C2.row_offsets) and np.array_equal(C.col_ids, C2.col_ids)
+    log(f"fat: nnz {C.nnz}, products {st['stats']['intermediate_product']}, err {worst:.2e}, "
+        f"timing {st['timing']['numeric']['fat']:.3f} ms")
+
+
 def test_rectangular_f32(gpu):
     log("case: rectangular 2^14 x 2^17 poisson(8) f32, A*A^T")
     A = gpu.CSR.rect_band(1 << 14, 1 << 17, 8.0, 0.3, 4096, 8, vtype=1, value_dist=1, seed=2)
