@@ -207,4 +207,21 @@ int launch_fat_compact(int vtype, const double *dense, const uint32_t *bits, uin
                        uint64_t b_cols, const uint64_t *c_off, uint32_t *c_cols, void *c_vals,
                        hipStream_t s);
 
+// Spy-plot segmentation (k_seg.hip).
+struct SegSums {
+  unsigned long long sum_products, sum_a_nnz, sum_sq_products;
+  uint32_t max_b_row, max_a_row, min_left, max_right;
+};
+int launch_seg_density(const DevCsr &A, uint32_t grid_h, uint32_t grid_w, uint32_t tile_rows,
+                       uint32_t tile_cols, uint32_t *density, hipStream_t s);
+int launch_seg_threshold(const uint32_t *density, uint32_t n, double k, uint32_t *labels,
+                         unsigned long long *out, hipStream_t s);
+int launch_seg_components(uint32_t grid_h, uint32_t grid_w, uint32_t max_sweeps,
+                          uint32_t *labels, uint32_t *next, unsigned long long *out,
+                          hipStream_t s);
+int launch_seg_dominant(const uint32_t *labels, uint32_t grid_h, uint32_t grid_w,
+                        uint32_t *row_label, uint32_t *row_hot, hipStream_t s);
+int launch_seg_reduce(const uint32_t *seg_start, const uint32_t *seg_end, uint32_t nseg,
+                      const DevCsr &A, const RowInfo &ri, SegSums *out, hipStream_t s);
+
 }  // namespace oc

@@ -177,6 +177,21 @@ struct oc_spgemm {
   uint32_t max_products = 0, max_a_nnz = 0;
   uint32_t bin1[kBinSlots + 1] = {0}, bin2[kBinSlots + 1] = {0};
   bool have_bin1 = false, have_bin2 = false;
+  // spy-plot segmentation of the last run (SpGEMM.cuh:1853-1886)
+  struct SegRecord {
+    uint32_t row_start, row_end, col_left, col_right, cls;
+    uint64_t sum_products, sum_a_nnz;
+    uint32_t max_b_row_len, max_a_row_len;
+    double mean_row_len, var_row_len, local_er;
+  };
+  struct Segmentation {
+    bool ran = false;
+    uint32_t grid_h = 0, grid_w = 0, tile_rows = 0, tile_cols = 0, num_components = 0;
+    uint64_t cc_iterations = 0;
+    double density_mean = 0, density_stddev = 0;
+    float t_density = 0, t_threshold = 0, t_cc = 0, t_classify = 0;
+    std::vector<SegRecord> segments;
+  } seg;
   uint64_t total_est = 0;
   uint64_t est_spare_need = 0;       // entries the overflow rows reserved
   bool est_separate_spare = false;   // ... in a buffer of their own
@@ -549,6 +564,7 @@ struct Run {
   int prologue_and_analysis();
   int sampling();
   int learned_capacities(bool *use_est);
+  int segmentation();
   int run_fat();
   int run_usparse();
   int run_precise();
@@ -648,6 +664,158 @@ int Run::learned_capacities(bool *use_est) {
   *use_est = (double)h->total_products >=
              c.learned_min_compression * (double)std::max<uint64_t>(h->learned_total, 1);
   mk.mark("est_hll_merge");
+  return OC_OK;
+}
+
+// Spy-plot segmentation of A (Segmentation.cuh:374-673): analysis only,
+// reported in the stats JSON.  Grid sizes, thresholds, the bounded label
+// propagation and flattening, the run-length encoding of the per-grid-row
+// labels, the merge of short bands and the class rule follow the reference;
+// the propagation sweeps are synchronous (deterministic) and dominant-label
+// ties go to the smaller label.
+int Run::segmentation() {
+  oc_spgemm::Segmentation &sg = h->seg;
+  sg = oc_spgemm::Segmentation();
+  sg.ran = true;
+  const uint64_t m = A.rows;
+  if (m == 0 || B.cols == 0) return OC_OK;
+  const uint64_t gmax = (uint64_t)std::max(8, c.segmentation_grid_max);
+  OC_REQUIRE(gmax <= 4096, "segmentation_grid_max above 4096");
+  sg.grid_h = (uint32_t)std::min<uint64_t>(gmax, m);
+  sg.grid_w = (uint32_t)std::min<uint64_t>(gmax, B.cols);
+  sg.tile_rows = (uint32_t)((m + sg.grid_h - 1) / sg.grid_h);
+  sg.tile_cols = (uint32_t)((B.cols + sg.grid_w - 1) / sg.grid_w);
+  const uint32_t n = sg.grid_h * sg.grid_w;
+  hipEvent_t ev[5];
+  for (hipEvent_t &e : ev) HIP_TRY(hipEventCreate(&e));
+  struct Ev {
+    hipEvent_t *e;
+    ~Ev() {
+      for (int i = 0; i < 5; ++i) (void)hipEventDestroy(e[i]);
+    }
+  } ev_guard{ev};
+  uint32_t *density, *labels, *next, *row_label, *row_hot;
+  unsigned long long *out;
+  OC_TRY(ar.get(&density, n));
+  OC_TRY(ar.get(&labels, n));
+  OC_TRY(ar.get(&next, n));
+  OC_TRY(ar.get(&row_label, sg.grid_h));
+  OC_TRY(ar.get(&row_hot, sg.grid_h));
+  OC_TRY(ar.get(&out, 4));
+  HIP_TRY(hipMemsetAsync(density, 0, (size_t)n * 4, s));
+  HIP_TRY(hipEventRecord(ev[0], s));
+  OC_TRY(launch_seg_density(A, sg.grid_h, sg.grid_w, sg.tile_rows, sg.tile_cols, density, s));
+  HIP_TRY(hipEventRecord(ev[1], s));
+  OC_TRY(launch_seg_threshold(density, n, c.segmentation_hot_threshold_k, labels, out, s));
+  HIP_TRY(hipEventRecord(ev[2], s));
+  OC_TRY(launch_seg_components(sg.grid_h, sg.grid_w,
+                               (uint32_t)std::max(0, c.segmentation_cc_max_iters), labels, next,
+                               out, s));
+  HIP_TRY(hipEventRecord(ev[3], s));
+  OC_TRY(launch_seg_dominant(labels, sg.grid_h, sg.grid_w, row_label, row_hot, s));
+  unsigned long long ho[3] = {0, 0, 0};
+  std::vector<uint32_t> band_label(sg.grid_h);
+  OC_TRY(readback(ho, out, 3, s));
+  OC_TRY(readback(band_label.data(), row_label, sg.grid_h, s));
+  const double mean = (double)ho[0] / (double)n;
+  double var = (double)ho[1] / (double)n - mean * mean;
+  if (var < 0.0) var = 0.0;
+  sg.density_mean = mean;
+  sg.density_stddev = std::sqrt(var);
+  sg.cc_iterations = ho[2];
+  // run-length encoding of the grid-row labels into row bands
+  struct Band {
+    uint32_t start, end, label;
+  };
+  std::vector<Band> bands;
+  uint32_t cur = band_label[0], cur_g = 0;
+  auto push = [&](uint32_t g_begin, uint32_t g_end, uint32_t label) {
+    const uint64_t b0 = (uint64_t)g_begin * sg.tile_rows;
+    const uint64_t b1 = std::min<uint64_t>(m, (uint64_t)g_end * sg.tile_rows);
+    if (b1 > b0) bands.push_back(Band{(uint32_t)b0, (uint32_t)b1, label});
+  };
+  for (uint32_t g = 1; g < sg.grid_h; ++g)
+    if (band_label[g] != cur) {
+      push(cur_g, g, cur);
+      cur = band_label[g];
+      cur_g = g;
+    }
+  {
+    const uint64_t b0 = (uint64_t)cur_g * sg.tile_rows;
+    if (m > b0) bands.push_back(Band{(uint32_t)b0, (uint32_t)m, cur});
+  }
+  // bands shorter than segmentation_min_segment_rows join the previous one
+  const uint32_t min_rows = (uint32_t)std::max(1, c.segmentation_min_segment_rows);
+  std::vector<Band> merged;
+  for (const Band &b : bands) {
+    if (!merged.empty() && b.end - b.start < min_rows) merged.back().end = b.end;
+    else merged.push_back(b);
+  }
+  std::vector<uint32_t> uniq;
+  for (const Band &b : merged)
+    if (b.label != 0xFFFFFFFFu && std::find(uniq.begin(), uniq.end(), b.label) == uniq.end())
+      uniq.push_back(b.label);
+  sg.num_components = (uint32_t)uniq.size();
+  if (merged.empty()) merged.push_back(Band{0, (uint32_t)m, 0xFFFFFFFFu});
+  const uint32_t nseg = (uint32_t)merged.size();
+  std::vector<uint32_t> hs(2 * (size_t)nseg);
+  for (uint32_t i = 0; i < nseg; ++i) {
+    hs[i] = merged[i].start;
+    hs[nseg + i] = merged[i].end;
+  }
+  uint32_t *dseg;
+  SegSums *dsum;
+  OC_TRY(ar.get(&dseg, 2 * (uint64_t)nseg));
+  OC_TRY(ar.get(&dsum, nseg));
+  HIP_TRY(hipMemcpyAsync(dseg, hs.data(), hs.size() * 4, hipMemcpyHostToDevice, s));
+  OC_TRY(launch_seg_reduce(dseg, dseg + nseg, nseg, A, ri, dsum, s));
+  std::vector<SegSums> sums(nseg);
+  OC_TRY(readback(sums.data(), dsum, nseg, s));
+  HIP_TRY(hipEventRecord(ev[4], s));
+  HIP_TRY(hipEventSynchronize(ev[4]));
+  (void)hipEventElapsedTime(&sg.t_density, ev[0], ev[1]);
+  (void)hipEventElapsedTime(&sg.t_threshold, ev[1], ev[2]);
+  (void)hipEventElapsedTime(&sg.t_cc, ev[2], ev[3]);
+  (void)hipEventElapsedTime(&sg.t_classify, ev[3], ev[4]);
+  // class rule (Segmentation.cuh:597-640): banded > dense > power law
+  for (uint32_t i = 0; i < nseg; ++i) {
+    const SegSums &t = sums[i];
+    oc_spgemm::SegRecord rec{};
+    rec.row_start = merged[i].start;
+    rec.row_end = merged[i].end;
+    rec.col_left = t.min_left == 0xFFFFFFFFu ? 0u : t.min_left;
+    rec.col_right = t.max_right;
+    rec.sum_products = t.sum_products;
+    rec.sum_a_nnz = t.sum_a_nnz;
+    rec.max_b_row_len = t.max_b_row;
+    rec.max_a_row_len = t.max_a_row;
+    const uint32_t rows = rec.row_end - rec.row_start;
+    const double mean_p = rows ? (double)t.sum_products / (double)rows : 0.0;
+    double var_p = 0.0;
+    if (rows) {
+      var_p = (double)t.sum_sq_products / (double)rows - mean_p * mean_p;
+      if (var_p < 0.0) var_p = 0.0;
+    }
+    const double cv = mean_p > 0.0 ? std::sqrt(var_p) / mean_p : 0.0;
+    const double er = t.sum_a_nnz ? (double)t.sum_products / (double)t.sum_a_nnz : 0.0;
+    const double span_ratio =
+        (double)(uint32_t)(rec.col_right + 1u - rec.col_left) / (double)B.cols;
+    const double row_density = span_ratio > 0.0 ? mean_p / ((double)B.cols * span_ratio) : 0.0;
+    rec.mean_row_len = mean_p;
+    rec.var_row_len = var_p;
+    rec.local_er = er;
+    uint32_t cls = 0;  // sparse uniform
+    if (span_ratio > 0.0 && span_ratio < 0.05 && mean_p > 0.0 &&
+        er >= (double)c.estimation_input_expansion_threshold)
+      cls = 3;  // banded
+    else if (row_density >= 0.25)
+      cls = 1;  // dense
+    else if (cv >= 1.5)
+      cls = 2;  // power law
+    rec.cls = cls;
+    sg.segments.push_back(rec);
+  }
+  mk.mark("segmentation");
   return OC_OK;
 }
 
@@ -1044,6 +1212,28 @@ struct Json {
     o << "}";
     first = false;
   }
+  void str(const char *k, const char *v) {
+    key(k);
+    o << "\"" << v << "\"";
+  }
+  // a list of objects: list_open, then item_open ... item_close per item
+  bool first_item = true;
+  void list_open(const char *k) {
+    key(k);
+    o << "[";
+    first_item = true;
+  }
+  void item_open() {
+    if (!first_item) o << ",";
+    first_item = false;
+    o << "{";
+    first = true;
+  }
+  void item_close() { o << "}"; }
+  void list_close() {
+    o << "]";
+    first = false;
+  }
   void arr(const char *k, const uint32_t *v, int n) {
     key(k);
     o << "[";
@@ -1128,6 +1318,43 @@ std::string stats_json(const oc_spgemm *h) {
     j.open("ultra_sparse");
     j.u64("kernel_size", d.spark_kernel_size);
     j.u64("outlier_size", d.spark_outlier_kernel_size);
+    j.close();
+  }
+  if (h->seg.ran) {  // SpGEMM.cuh:1853-1886
+    static const char *kClass[4] = {"SPARSE_UNIFORM", "DENSE", "POWER_LAW", "BANDED"};
+    const oc_spgemm::Segmentation &sg = h->seg;
+    j.open("segmentation");
+    j.u64("grid_h", sg.grid_h);
+    j.u64("grid_w", sg.grid_w);
+    j.u64("tile_rows", sg.tile_rows);
+    j.u64("tile_cols", sg.tile_cols);
+    j.num("density_mean", sg.density_mean);
+    j.num("density_stddev", sg.density_stddev);
+    j.u64("num_components", sg.num_components);
+    j.u64("cc_iterations", sg.cc_iterations);
+    j.num("time_density_ms", sg.t_density);
+    j.num("time_threshold_ms", sg.t_threshold);
+    j.num("time_cc_ms", sg.t_cc);
+    j.num("time_classify_ms", sg.t_classify);
+    j.list_open("segments");
+    for (const oc_spgemm::SegRecord &r : sg.segments) {
+      j.item_open();
+      j.u64("row_start", r.row_start);
+      j.u64("row_end", r.row_end);
+      j.u64("row_count", r.row_end - r.row_start);
+      j.u64("col_left", r.col_left);
+      j.u64("col_right", r.col_right);
+      j.str("class", kClass[r.cls & 3]);
+      j.u64("sum_products", r.sum_products);
+      j.u64("sum_a_nnz", r.sum_a_nnz);
+      j.num("mean_row_len", r.mean_row_len);
+      j.num("var_row_len", r.var_row_len);
+      j.num("local_er", r.local_er);
+      j.u64("max_b_row_len", r.max_b_row_len);
+      j.u64("max_a_row_len", r.max_a_row_len);
+      j.item_close();
+    }
+    j.list_close();
     j.close();
   }
   if (h->cfg.track_stage_times) {
@@ -1285,6 +1512,9 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
     h->d = Decision();
   } else {
     rc = r.prologue_and_analysis();
+    h->seg = oc_spgemm::Segmentation();
+    if (rc == OC_OK && h->cfg.use_segmented_workflow && h->d.ana1_type != 2)
+      rc = r.segmentation();
     if (rc == OC_OK && h->keep_plan) {
       h->d.ana1_type = 1;  // reported as the precise workflow
       h->d.ana2_type = 0;
