@@ -92,6 +92,8 @@ __global__ void k_batch_extract(const uint64_t *__restrict__ rp, const uint32_t 
 
 int spgemm_replay_async(oc_spgemm *h, hipStream_t s);
 void spgemm_drop_plan(oc_spgemm *h);
+void spgemm_set_blocks(oc_spgemm *h, const uint32_t *b_block_base, uint64_t span,
+                       const void *b_row_offsets);
 
 }  // namespace oc
 
@@ -106,6 +108,7 @@ struct oc_batch {
   std::vector<uint64_t> row_base, col_base;  // count + 1 each
   uint64_t a_total_nnz = 0, b_total_nnz = 0;
   BatchDesc *dA = nullptr, *dB = nullptr;
+  uint32_t *d_bbase = nullptr;  // per row of the concatenated B: its block's first column
   bool planned = false;
   hipStream_t own = nullptr;
   hipGraph_t graph = nullptr;
@@ -128,9 +131,12 @@ struct oc_batch {
     if (!b_is_a) oc_csr_free(&B);
     std::memset(&B, 0, sizeof(B));
     oc_csr_free(&A);
+    spgemm_set_blocks(h, nullptr, 0, nullptr);
     if (dA) (void)hipFree(dA);
     if (dB) (void)hipFree(dB);
+    if (d_bbase) (void)hipFree(d_bbase);
     dA = dB = nullptr;
+    d_bbase = nullptr;
     count = 0;
   }
 };
@@ -226,6 +232,21 @@ extern "C" int oc_batch_set_inputs(oc_batch *b, const oc_csr *As, const oc_csr *
     HIP_TRY(hipMalloc((void **)&b->dB, sizeof(BatchDesc) * count));
     HIP_TRY(hipMemcpy(b->dB, db.data(), sizeof(BatchDesc) * count, hipMemcpyHostToDevice));
     OC_TRY(concat(b, b->dB, b->B, bnnz, false, s));
+  }
+  // B_i's rows hold only columns of block i: tiles of the long-row kernel
+  // are counted from each block's first column (tile tables then span the
+  // widest block, not all of the concatenated columns)
+  {
+    std::vector<uint32_t> hb(acols ? acols : 1);
+    uint64_t span = 0, r = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+      for (uint64_t j = 0; j < As[i].cols; ++j) hb[r + j] = (uint32_t)cb[i];
+      r += As[i].cols;
+      span = std::max<uint64_t>(span, cb[i + 1] - cb[i]);
+    }
+    HIP_TRY(hipMalloc((void **)&b->d_bbase, hb.size() * 4));
+    HIP_TRY(hipMemcpy(b->d_bbase, hb.data(), hb.size() * 4, hipMemcpyHostToDevice));
+    spgemm_set_blocks(b->h, b->d_bbase, span, b->B.row_offsets);
   }
   HIP_TRY(hipStreamSynchronize(s));
   return OC_OK;

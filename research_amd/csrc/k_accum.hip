@@ -784,12 +784,14 @@ static int zero_u32(uint32_t *p, uint32_t n, hipStream_t s) {
 __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t *__restrict__ b_rp,
                                                      const uint32_t *__restrict__ b_ci,
                                                      uint32_t rows, uint32_t logW,
-                                                     uint32_t T1, uint32_t *__restrict__ ts) {
+                                                     uint32_t T1, const uint32_t *__restrict__ base,
+                                                     uint32_t *__restrict__ ts) {
   const int lane = lane_id();
   for (uint32_t k = blockIdx.x * 4 + threadIdx.x / kWave; k < rows; k += gridDim.x * 4) {
     const uint32_t s = b_rp[k], e = b_rp[k + 1];
+    const uint64_t b0 = base ? base[k] : 0;  // tiles counted from the row's block
     for (uint32_t t = lane; t < T1; t += kWave) {
-      const uint64_t bound = (uint64_t)t << logW;
+      const uint64_t bound = b0 + ((uint64_t)t << logW);
       uint32_t lo = s, hi = e;
       while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -1008,7 +1010,8 @@ template <typename T, int BS>
 __global__ __launch_bounds__(BS) void k_num_rank(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
     uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1,
-    const uint32_t *__restrict__ tsw, uint32_t cap, uint32_t pcap, KOut<T> o) {
+    const uint32_t *__restrict__ tsw, const uint32_t *__restrict__ bbase, uint32_t cap,
+    uint32_t pcap, KOut<T> o) {
   extern __shared__ __align__(16) uint8_t smem[];
   uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
   uint32_t *sk = reinterpret_cast<uint32_t *>(smem + 256);
@@ -1055,7 +1058,10 @@ __global__ __launch_bounds__(BS) void k_num_rank(
     const uint64_t rcap = o.cap ? (uint64_t)o.cap[r] : ~0ull;
     bool overflow = false;
     uint64_t total = 0;
-    const uint32_t t_begin = left >> lw, t_end = hi >> lw;
+    // tiles are counted from the row's column block (block-diagonal batches:
+    // every B row a row references starts the same block), else from 0
+    const uint32_t base_r = bbase ? bbase[a.a_ci[a0]] : 0u;
+    const uint32_t t_begin = (left - base_r) >> lw, t_end = (hi - base_r) >> lw;
     const uint32_t nwords = 1u << (lw - 5);
     const uint32_t wpt = (nwords + BS - 1) / BS;
     const uint32_t b0 = min(nwords, tid * wpt), b1 = min(nwords, b0 + wpt);
@@ -1142,7 +1148,7 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       }
     }
     for (uint32_t t = t_begin; t <= t_end; ++t) {
-      const uint64_t wbase = (uint64_t)t << lw;
+      const uint64_t wbase = (uint64_t)base_r + ((uint64_t)t << lw);
       // ---- pass 1: mark the output columns
       uint32_t any = 0, Plast = 0;
       bool keep = false;
@@ -1287,8 +1293,8 @@ int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row) {
 }
 
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, uint32_t pcap,
-                    uint32_t bs, const NumOut &o, void *scratch, hipStream_t s) {
+                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, const uint32_t *bbase,
+                    uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s) {
   if (nrows == 0) return OC_OK;
   const Tables &t = tables();
   uint32_t *counter = (uint32_t *)scratch;
@@ -1308,7 +1314,8 @@ int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint
     const size_t lds = fixed + (size_t)cap * 12;                                    \
     OC_TRY(set_lds(k_num_rank<T, B>, lds));                                         \
     hipLaunchKernelGGL((k_num_rank<T, B>), dim3(grid), dim3(B), lds, s, kargs<T>(a), \
-                       rows, nrows, counter, logT, ts, T1, tsw, cap, pcap, kout<T>(o)); \
+                       rows, nrows, counter, logT, ts, T1, tsw, bbase, cap, pcap,         \
+                       kout<T>(o));                                                          \
   } while (0)
   if (a.A.vtype == OC_F64) {
     if (bs == 256) GO(double, 256);
@@ -1340,10 +1347,10 @@ int tile_plan(uint64_t b_rows, uint64_t b_cols, uint32_t log2_cols, uint32_t bs,
 }
 
 int launch_tile_starts(const DevCsr &B, uint32_t logW, uint32_t T1, uint32_t *ts,
-                       hipStream_t s) {
+                       hipStream_t s, const uint32_t *base) {
   if (B.rows == 0) return OC_OK;
   hipLaunchKernelGGL(k_tile_starts, dim3(std::min<uint32_t>(cdiv(B.rows, 4), 1u << 20)),
-                     dim3(256), 0, s, B.rp, B.ci, (uint32_t)B.rows, logW, T1, ts);
+                     dim3(256), 0, s, B.rp, B.ci, (uint32_t)B.rows, logW, T1, base, ts);
   LAUNCH_CHECK();
   return OC_OK;
 }

@@ -158,8 +158,9 @@ int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
 // tile-start table (B.rows x T1 u32); returns 0 when tiles do not apply.
 int tile_plan(uint64_t b_rows, uint64_t b_cols, uint32_t log2_cols, uint32_t bs,
               uint32_t *logW, uint32_t *T1, size_t *ts_bytes);
+// base (optional, per B row): tiles are counted from base[k] instead of 0
 int launch_tile_starts(const DevCsr &B, uint32_t logW, uint32_t T1, uint32_t *ts,
-                       hipStream_t s);
+                       hipStream_t s, const uint32_t *base = nullptr);
 int launch_num_tiles(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logW,
                      uint32_t bs, const uint32_t *ts, uint32_t T1, const NumOut &o,
                      void *scratch, hipStream_t s, const uint32_t *nrows_dev = nullptr);
@@ -168,9 +169,11 @@ int launch_num_tiles(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uin
 // entries per B row, built by launch_tile_starts at rank_wide_log(bs)).
 uint32_t rank_wide_log(uint32_t bs);
 int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row);
+// bbase (optional, per B row): the first column of the row's block in a
+// block-diagonal product; the tile tables are then built from the same bases.
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, uint32_t pcap,
-                    uint32_t bs, const NumOut &o, void *scratch, hipStream_t s);
+                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, const uint32_t *bbase,
+                    uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s);
 // Ultra-sparse expand-sort-compress for all rows with products <= esc_n.
 int launch_esc(int esc_n, const AccArgs &a, const NumOut &o, hipStream_t s);
 

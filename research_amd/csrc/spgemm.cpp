@@ -177,6 +177,11 @@ struct oc_spgemm {
   uint32_t max_products = 0, max_a_nnz = 0;
   uint32_t bin1[kBinSlots + 1] = {0}, bin2[kBinSlots + 1] = {0};
   bool have_bin1 = false, have_bin2 = false;
+  // block-diagonal B (set by the batched layer for its runs): first column
+  // of each B row's block and the widest block
+  const uint32_t *b_block_base = nullptr;
+  uint64_t b_block_span = 0;
+  const void *b_block_rp = nullptr;  // the B the bases belong to
   // spy-plot segmentation of the last run (SpGEMM.cuh:1853-1886)
   struct SegRecord {
     uint32_t row_start, row_end, col_left, col_right, cls;
@@ -226,6 +231,7 @@ struct oc_spgemm {
     uint32_t hoff[kBinSlots + 1] = {0};
     NumOut o{};
     uint32_t *tile_ts = nullptr, *tile_tsw = nullptr;
+    const uint32_t *block_base = nullptr;
     uint32_t tile_logw = 0, tile_t1 = 0, max_a_nnz = 0;
     int tiles_ok = -1;
     void *win_scratch = nullptr;
@@ -357,6 +363,7 @@ struct Run {
   uint32_t *learned_est = nullptr;  // learned capacities (learned_capacities)
   void *win_scratch = nullptr;
   uint32_t *tile_ts = nullptr, *tile_tsw = nullptr;
+  const uint32_t *block_base = nullptr;  // per-B-row block bases of the tables
   uint32_t tile_logw = 0, tile_t1 = 0;
   int tiles_ok = -1;
   // long rows in column bands (precise workflow)
@@ -401,21 +408,30 @@ struct Run {
     return OC_OK;
   }
 
-  // Tile-start table of B for the tiled numeric kernel (built once).
+  // Tile-start table of B for the tiled numeric kernel (built once).  For
+  // a block-diagonal product (set by the batched layer) the tiles of each
+  // B row are counted from its block's first column, so the table spans the
+  // widest block instead of all of B's columns; only the rank-tile kernel
+  // reads such a table.
   int tiles() {
     if (tiles_ok >= 0) return OC_OK;
     size_t bytes = 0;
-    tiles_ok = (h->cfg.use_tiled_numeric != 0) &&
-               tile_plan(B.rows, B.cols, (uint32_t)h->cfg.tile_log2_cols,
+    // the bases apply only to the B they were set for
+    const bool blocks = h->b_block_base != nullptr && h->b_block_rp == (const void *)B.rp;
+    block_base = blocks ? h->b_block_base : nullptr;
+    const uint64_t width = blocks ? h->b_block_span : B.cols;
+    tiles_ok = (h->cfg.use_tiled_numeric == 2 || (h->cfg.use_tiled_numeric == 1 && !blocks)) &&
+               tile_plan(B.rows, width, (uint32_t)h->cfg.tile_log2_cols,
                          (uint32_t)h->cfg.tile_block, &tile_logw, &tile_t1, &bytes);
     if (tiles_ok) {
       OC_TRY(ar.get(&tile_ts, bytes / 4));
-      OC_TRY(launch_tile_starts(B, tile_logw, tile_t1, tile_ts, s));
+      OC_TRY(launch_tile_starts(B, tile_logw, tile_t1, tile_ts, s, block_base));
       size_t per = 0;
       const uint32_t bs = (uint32_t)h->cfg.tile_block;
-      if (h->cfg.use_tiled_numeric == 2 && wide_starts_plan(B.cols, bs, &per)) {
+      if (h->cfg.use_tiled_numeric == 2 && wide_starts_plan(width, bs, &per)) {
         OC_TRY(ar.get(&tile_tsw, B.rows * per));
-        OC_TRY(launch_tile_starts(B, rank_wide_log(bs), (uint32_t)per, tile_tsw, s));
+        OC_TRY(launch_tile_starts(B, rank_wide_log(bs), (uint32_t)per, tile_tsw, s,
+                                  block_base));
       }
       mk.mark("numeric_tile_starts");
     }
@@ -428,7 +444,7 @@ struct Run {
     OC_TRY(tiles());
     if (tiles_ok && h->cfg.use_tiled_numeric == 2)
       return launch_num_rank(acc(), rows, n, tile_logw, tile_ts, tile_t1, tile_tsw,
-                             (uint32_t)h->cfg.tile_products_cap,
+                             block_base, (uint32_t)h->cfg.tile_products_cap,
                              (uint32_t)h->cfg.tile_block, o,
                              win_scratch, s);
     if (tiles_ok)
@@ -1018,6 +1034,7 @@ int Run::run_precise() {
     p.o = o;
     p.tile_ts = tile_ts;
     p.tile_tsw = tile_tsw;
+    p.block_base = block_base;
     p.tile_logw = tile_logw;
     p.tile_t1 = tile_t1;
     p.tiles_ok = tiles_ok;
@@ -1602,6 +1619,7 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   r.ri = p.ri;
   r.tile_ts = p.tile_ts;
   r.tile_tsw = p.tile_tsw;
+  r.block_base = p.block_base;
   r.tile_logw = p.tile_logw;
   r.tile_t1 = p.tile_t1;
   r.tiles_ok = p.tiles_ok;
@@ -1631,6 +1649,14 @@ namespace oc {
 // Used by the batched layer (batch.hip) to record the numeric phase into
 // its own graph after refreshing the concatenated values.
 int spgemm_replay_async(oc_spgemm *h, hipStream_t s) { return replay_impl(h, s); }
+void spgemm_set_blocks(oc_spgemm *h, const uint32_t *b_block_base, uint64_t span,
+                       const void *b_row_offsets) {
+  if (!h) return;
+  h->drop_plan();  // a kept plan's tables were built for the previous B
+  h->b_block_base = b_block_base;
+  h->b_block_span = b_block_base ? span : 0;
+  h->b_block_rp = b_block_base ? b_row_offsets : nullptr;
+}
 void spgemm_drop_plan(oc_spgemm *h) {
   if (h) h->drop_plan();
 }

@@ -119,3 +119,48 @@ def test_batch_graph_refresh(gpu):
         v2, _, _ = b.result()
         assert np.array_equal(v1.row_offsets(), v2.row_offsets())
         log(f"graph refresh {it}: ok")
+
+
+@pytest.mark.parametrize("mode", ["run", "graph", "learned_est"])
+def test_batch_long_rows_block_tiles(gpu, mode):
+    """A batch whose products have rows beyond the LDS kernels: the
+    rank-tile kernel takes them with tiles counted from each product's
+    column block (the concatenated B is far wider than one block).  Each
+    product equals the oracle's, for a plain run, a graph replay after new
+    values, and the estimation workflow with every estimate set to 1 (all
+    long rows overflow and are recomputed by the same kernel)."""
+    log(f"case: batch with long rows ({mode})")
+    mats = [gpu.CSR.rmat(14, 32 << 14, seed=60 + i) for i in range(2)]
+    mats += [gpu.CSR.erdos_renyi(3000, 6.0, seed=70), gpu.CSR.rmat(13, 16 << 13, seed=71)]
+    dev = [M.upload() for M in mats]
+    b = gpu.Batch(gpu.Config(track_stage_times=1))
+    b.set_inputs(dev)
+    if mode == "graph":
+        b.plan()
+        b.capture()
+        rng = np.random.default_rng(3)
+        for d, M in zip(dev, mats):
+            _set_values(gpu, d, M, rng.uniform(-1, 1, M.nnz))
+        b.launch()
+    elif mode == "learned_est":
+        rows = sum(M.rows for M in mats)
+        est = gpu.DeviceBuffer.from_array(np.ones(rows, np.uint32))
+        h = gpu.lib.oc_batch_handle(b.h)
+        gpu._check(gpu.lib.oc_spgemm_set_workflow(h, 1))
+        gpu._check(gpu.lib.oc_spgemm_set_estimates(h, est.ptr))
+        b.run()
+        gpu._check(gpu.lib.oc_spgemm_set_estimates(h, None))
+    else:
+        b.run()
+    st = b.get_stats()
+    if mode != "graph":
+        assert st["binning_2"][14] - st["binning_2"][13] > 0 or mode == "learned_est"
+        fam = set(st.get("kernels", {}))
+        log(f"{mode}: workflow {st['decision']['workflow']}, long rows "
+            f"{st['binning_2'][14] - st['binning_2'][13]}, families {sorted(fam)}")
+    worst = 0.0
+    for i, M in enumerate(mats):
+        Ci = b.extract(i).download()
+        w, _ = check_product(M, M, Ci, 1e-12, f"{mode} product {i}")
+        worst = max(worst, w)
+    log(f"{mode}: max normalised error {worst:.2e}")
