@@ -539,14 +539,16 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
   }
   if (ovf) hdr[1] = 1;
   __syncthreads();
-  // compaction of occupied slots
+  // compaction of occupied slots; keys relative to the row's first column,
+  // so the sort needs only the bits of the row's column span
+  const uint32_t lo = a.left[r];
   if (!hdr[1]) {
     for (uint32_t i = threadIdx.x; i < S; i += BS) {
       const uint32_t kk = keys[i];
       if (kk != kEmpty) {
         const uint32_t pos = atomicAdd(&hdr[2], 1u);
         if (pos < L.nmax) {
-          ck[pos] = kk;
+          ck[pos] = kk - lo;
           ci[pos] = (uint16_t)i;
         }
       }
@@ -561,12 +563,15 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
   }
   uint32_t *k1 = keys;
   uint16_t *i1 = reinterpret_cast<uint16_t *>(keys + L.nmax);
-  const int parity = block_radix_sort(ck, ci, k1, i1, cnt, dsum, (int)n, key_bits);
+  // every output column lies in [left, right]; an empty row has n == 0
+  const uint32_t span = n ? a.right[r] - lo : 0u;
+  const int kb = min(key_bits, span ? 32 - __clz(span) : 0);
+  const int parity = block_radix_sort(ck, ci, k1, i1, cnt, dsum, (int)n, kb);
   const uint32_t *sk = parity ? k1 : ck;
   const uint16_t *si = parity ? i1 : ci;
   const uint64_t off = o.off[r];
   for (uint32_t i = threadIdx.x; i < n; i += BS) {
-    o.cols[off + i] = sk[i];
+    o.cols[off + i] = sk[i] + lo;
     o.vals[off + i] = (T)vals[si[i]];
   }
   if (o.nnz && threadIdx.x == 0) o.nnz[r] = n;
