@@ -94,6 +94,66 @@ __device__ __forceinline__ T block_excl_sum(T v, T *scratch, T *total) {
   return r;
 }
 
+// Inclusive scan over the 64 lanes of a wave with DPP moves: row shifts by
+// 1, 2, 4, 8 within each row of 16 lanes, then lane 15 of rows 0 and 2
+// into rows 1 and 3 and lane 31 into rows 2 and 3.  Lanes whose source is
+// outside the row take `id`.  All 64 lanes must be active.
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t id, Op op) {
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, 0x111, 0xf, 0xf, false));
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, 0x112, 0xf, 0xf, false));
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, 0x114, 0xf, 0xf, false));
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, 0x118, 0xf, 0xf, false));
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, 0x142, 0xa, 0xf, false));
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, 0x143, 0xc, 0xf, false));
+  return x;
+}
+
+// The value of the lane below (lane 0 takes `id`): DPP wave shift right.
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t x, uint32_t id) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, 0x138, 0xf, 0xf, false);
+}
+
+// 32-bit block-wide exclusive sum (blockDim.x a multiple of 64, at most
+// 1024): DPP wave scans, one LDS slot per wave, every thread adds up the
+// slots of the waves before it.  `scratch` holds 16 entries.
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t *scratch,
+                                                   uint32_t *total) {
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+  const uint32_t incl =
+      wave_incl_scan(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+  if (lane == kWave - 1) scratch[wid] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < 16; ++w) {
+    const uint32_t s = scratch[w];  // in bounds; slots past nw are masked
+    all += w < nw ? s : 0u;
+    before += w < wid ? s : 0u;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+// Block-wide exclusive max of 32-bit values (identity 0), same shape.
+__device__ __forceinline__ uint32_t block_excl_max(uint32_t v, uint32_t *scratch) {
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  const uint32_t incl =
+      wave_incl_scan(v, 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+  if (lane == kWave - 1) scratch[wid] = incl;
+  __syncthreads();
+  uint32_t before = wave_shr1(incl, 0u);
+#pragma unroll
+  for (int w = 0; w < 16; ++w) {
+    const uint32_t s = scratch[w];
+    before = (w < wid && s > before) ? s : before;
+  }
+  __syncthreads();
+  return before;
+}
+
 // Block-wide exclusive max (identity 0) for blockDim.x <= 1024; `scratch`
 // holds 16 entries.  Returns the max over the threads before this one.
 template <typename T>

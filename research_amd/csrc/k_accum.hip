@@ -979,27 +979,14 @@ struct RankShape {
   }
 };
 
-// OR the column bits of up to 8 products into the tile bitmap, combining
-// products that fall into the same 32-column word.
+// OR the column bits of up to 8 products into the tile bitmap.
 __device__ __forceinline__ void mark8(const uint32_t (&col)[8], uint64_t wbase, uint32_t *bits) {
-  uint32_t cw = kEmpty, cm = 0;
 #pragma unroll
   for (int u = 0; u < 8; ++u)
     if (col[u] != kEmpty) {
       const uint32_t x = (uint32_t)(col[u] - wbase);
-      if ((x >> 5) != cw) {
-        if (cm) atomicOr(&bits[cw], cm);
-        cw = x >> 5;
-        cm = 0;
-      }
-      cm |= 1u << (x & 31);
+      atomicOr(&bits[x >> 5], 1u << (x & 31));
     }
-  if (cm) atomicOr(&bits[cw], cm);
-}
-
-__device__ __forceinline__ uint32_t rank_of(uint32_t x, const uint32_t *wpre, const uint32_t *bits) {
-  const uint32_t wd = x >> 5;
-  return wpre[wd] + __popc(bits[wd] & ((1u << (x & 31)) - 1u));
 }
 
 // One workgroup per long row (rows taken from an atomic counter).  A
@@ -1031,6 +1018,9 @@ __global__ __launch_bounds__(BS) void k_num_rank(
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < RankShape<BS>::WORDS; i += BS) bits[i] = 0;
   for (uint32_t i = tid; i < cap; i += BS) acc[i] = 0.0;
+  // owner entries are thread ids (< BS) from the start, so the unmasked
+  // reads past a round's end stay inside sbase
+  for (uint32_t i = tid; i < RankShape<BS>::ROUND; i += BS) own[i] = 0;
   __syncthreads();
   for (;;) {
     if (tid == 0) hdr[0] = atomicAdd(counter, 1u);
@@ -1131,17 +1121,37 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       if (8 * tid < n) *reinterpret_cast<us8 *>(own + 8 * tid) = v;
       __syncthreads();
     };
-    // products p0 + tid + 1024 u of the current map: B entry q[u], A entry w[u]
+    // products p0 + tid + BS u of the current map: B entry q[u] (kEmpty
+    // past n), A entry w[u].  All owner reads are issued before any sbase
+    // read, with no lane branches.
     auto locate = [&](uint32_t p0, uint32_t n, uint32_t (&q)[8], uint32_t (&w)[8]) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) w[u] = own[tid + (uint32_t)u * BS];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const uint32_t i = tid + (uint32_t)u * BS;
-        q[u] = kEmpty;
-        w[u] = 0;
-        if (i < n) {
-          const uint32_t ow = own[i];
-          q[u] = sbase[ow] + p0 + i;
-          w[u] = ow;
+        const uint32_t sb = sbase[w[u]];
+        q[u] = i < n ? sb + p0 + i : kEmpty;
+      }
+    };
+    // products (column, value) summed into ranks [r0, r1): all bitmap and
+    // prefix reads first, then the adds
+    auto accumulate = [&](const uint32_t (&col)[8], const double (&pv)[8], uint64_t wbase,
+                          uint32_t r0, uint32_t r1) {
+      uint32_t wp[8], bw[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t wd = col[u] != kEmpty ? (uint32_t)(col[u] - wbase) >> 5 : 0u;
+        wp[u] = wpre[wd];
+        bw[u] = bits[wd];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t x = (uint32_t)(col[u] - wbase);
+        const uint32_t rk = wp[u] + __popc(bw[u] & ((1u << (x & 31)) - 1u));
+        if (col[u] != kEmpty && rk >= r0 && rk < r1) {
+          lds_add_f64(&acc[rk - r0], pv[u]);
+          cmap[rk - r0] = col[u];
         }
       }
     };
@@ -1157,7 +1167,8 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       // ---- pass 1: mark the output columns
       uint32_t any = 0, Plast = 0;
       bool keep = false;
-      uint32_t kc[8];  // kept columns of round 0 (single-chunk rows)
+      uint32_t kc[8];  // kept columns and products of round 0 (single-chunk rows)
+      double kv[8];
       for (uint32_t c0 = a0; c0 < a1; c0 += RankShape<BS>::A) {
         const uint32_t nc = min(RankShape<BS>::A, a1 - c0);
         if (!one_chunk) stage(c0, nc);
@@ -1167,8 +1178,8 @@ __global__ __launch_bounds__(BS) void k_num_rank(
         Plast = P;
         keep = one_chunk;
         // single-chunk rows take the rounds last to first, so round 0 (the
-        // only full one when there are two) ends pass 1 with its owner map
-        // in LDS and its columns in registers, and pass 2 gathers it once
+        // only full one when there are two) ends pass 1 with its columns and
+        // products in registers, and pass 2 sums it without a gather
         const uint32_t nr = (P + RankShape<BS>::ROUND - 1) / RankShape<BS>::ROUND;
         for (uint32_t ri = 0; ri < nr; ++ri) {
           const uint32_t p0 = (one_chunk ? nr - 1 - ri : ri) * RankShape<BS>::ROUND;
@@ -1178,6 +1189,11 @@ __global__ __launch_bounds__(BS) void k_num_rank(
           locate(p0, n, q, w);
 #pragma unroll
           for (int u = 0; u < 8; ++u) kc[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
+          if (one_chunk && p0 == 0) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              kv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
+          }
           mark8(kc, wbase, bits);
         }
         if (!one_chunk) __syncthreads();
@@ -1214,63 +1230,47 @@ __global__ __launch_bounds__(BS) void k_num_rank(
           col[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
           pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
         }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (col[u] != kEmpty) {
-            const uint32_t rk = rank_of((uint32_t)(col[u] - wbase), wpre, bits);
-            if (rk >= r0 && rk < r1) {
-              lds_add_f64(&acc[rk - r0], pv[u]);
-              cmap[rk - r0] = col[u];
-            }
-          }
+        accumulate(col, pv, wbase, r0, r1);
       };
-      for (uint32_t r0 = 0; r0 < m; r0 += cap) {
-        const uint32_t r1 = min(m, r0 + cap);
-        if (keep) {
-          const uint32_t n0 = min(RankShape<BS>::ROUND, Plast);
-          if (r0 == 0 || Plast <= RankShape<BS>::ROUND) {
-            // round 0's owner map is still in LDS and its columns in
-            // registers: gather the values only
-            uint32_t q[8], w[8];
-            locate(0, n0, q, w);
-            double pv[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-              if (kc[u] != kEmpty) {
-                const uint32_t rk = rank_of((uint32_t)(kc[u] - wbase), wpre, bits);
-                if (rk >= r0 && rk < r1) {
-                  lds_add_f64(&acc[rk - r0], pv[u]);
-                  cmap[rk - r0] = kc[u];
-                }
-              }
-          } else {
-            round_sum(0, n0, r0, r1);  // a later rank range: round 0's map was replaced
+      // every round of the tile summed into ranks [r0, r1)
+      auto sum_rounds = [&](uint32_t r0, uint32_t r1) {
+        for (uint32_t c0 = a0; c0 < a1; c0 += RankShape<BS>::A) {
+          const uint32_t nc = min(RankShape<BS>::A, a1 - c0);
+          uint32_t P = Plast;
+          if (!one_chunk) {
+            stage(c0, nc);
+            bounds(t, nc, P);
           }
-          for (uint32_t p0 = RankShape<BS>::ROUND; p0 < Plast; p0 += RankShape<BS>::ROUND)
-            round_sum(p0, min(RankShape<BS>::ROUND, Plast - p0), r0, r1);
-        } else {
-          for (uint32_t c0 = a0; c0 < a1; c0 += RankShape<BS>::A) {
-            const uint32_t nc = min(RankShape<BS>::A, a1 - c0);
-            uint32_t P = Plast;
-            if (!one_chunk) {
-              stage(c0, nc);
-              bounds(t, nc, P);
-            }
-            for (uint32_t p0 = 0; p0 < P; p0 += RankShape<BS>::ROUND)
-              round_sum(p0, min(RankShape<BS>::ROUND, P - p0), r0, r1);
-            if (!one_chunk) __syncthreads();
-          }
+          for (uint32_t p0 = 0; p0 < P; p0 += RankShape<BS>::ROUND)
+            round_sum(p0, min(RankShape<BS>::ROUND, P - p0), r0, r1);
+          if (!one_chunk) __syncthreads();
         }
+      };
+      // coalesced write-out of ranks [r0, r1)
+      auto write_ranks = [&](uint32_t r0, uint32_t r1) {
         __syncthreads();
-        // coalesced write-out of ranks [r0, r1)
         for (uint32_t i = tid; i < r1 - r0; i += BS) {
           o.cols[off + total + r0 + i] = cmap[i];
           o.vals[off + total + r0 + i] = (T)acc[i];
           acc[i] = 0.0;
         }
         __syncthreads();
+      };
+      // the first rank range takes round 0 from registers; later ranges
+      // (more outputs than slots) gather every round again
+      const uint32_t r1a = min(m, cap);
+      if (keep) {
+        accumulate(kc, kv, wbase, 0, r1a);
+        for (uint32_t p0 = RankShape<BS>::ROUND; p0 < Plast; p0 += RankShape<BS>::ROUND)
+          round_sum(p0, min(RankShape<BS>::ROUND, Plast - p0), 0, r1a);
+      } else {
+        sum_rounds(0, r1a);
+      }
+      write_ranks(0, r1a);
+      for (uint32_t r0 = cap; r0 < m; r0 += cap) {
+        const uint32_t r1 = min(m, r0 + cap);
+        sum_rounds(r0, r1);
+        write_ranks(r0, r1);
       }
       for (uint32_t i = b0; i < b1; ++i) bits[i] = 0;
       total += m;
