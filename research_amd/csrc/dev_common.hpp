@@ -115,43 +115,41 @@ __device__ __forceinline__ uint32_t wave_shr1(uint32_t x, uint32_t id) {
 }
 
 // 32-bit block-wide exclusive sum (blockDim.x a multiple of 64, at most
-// 1024): DPP wave scans, one LDS slot per wave, every thread adds up the
-// slots of the waves before it.  `scratch` holds 16 entries.
+// 1024): DPP wave scans, one LDS slot per wave; every wave then scans the
+// slots across its lanes and reads its own prefix with readlane (no
+// per-wave masks, which the compiler would keep live in SGPRs).  `scratch`
+// holds 16 entries.
 __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t *scratch,
                                                    uint32_t *total) {
-  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int nw = blockDim.x / kWave;
-  const uint32_t incl =
-      wave_incl_scan(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+  auto add = [](uint32_t a, uint32_t b) { return a + b; };
+  const uint32_t incl = wave_incl_scan(v, 0u, add);
   if (lane == kWave - 1) scratch[wid] = incl;
   __syncthreads();
-  uint32_t before = 0, all = 0;
-#pragma unroll
-  for (int w = 0; w < 16; ++w) {
-    const uint32_t s = scratch[w];  // in bounds; slots past nw are masked
-    all += w < nw ? s : 0u;
-    before += w < wid ? s : 0u;
-  }
+  const uint32_t s = lane < nw ? scratch[lane] : 0u;
+  const uint32_t si = wave_incl_scan(s, 0u, add);
+  *total = (uint32_t)__builtin_amdgcn_readlane((int)si, kWave - 1);
+  const uint32_t before = (uint32_t)__builtin_amdgcn_readlane((int)(si - s), wid);
   __syncthreads();
-  *total = all;
   return before + incl - v;
 }
 
 // Block-wide exclusive max of 32-bit values (identity 0), same shape.
 __device__ __forceinline__ uint32_t block_excl_max(uint32_t v, uint32_t *scratch) {
-  const int lane = lane_id(), wid = threadIdx.x / kWave;
-  const uint32_t incl =
-      wave_incl_scan(v, 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+  const int lane = lane_id();
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int nw = blockDim.x / kWave;
+  auto mx = [](uint32_t a, uint32_t b) { return a > b ? a : b; };
+  const uint32_t incl = wave_incl_scan(v, 0u, mx);
   if (lane == kWave - 1) scratch[wid] = incl;
   __syncthreads();
-  uint32_t before = wave_shr1(incl, 0u);
-#pragma unroll
-  for (int w = 0; w < 16; ++w) {
-    const uint32_t s = scratch[w];
-    before = (w < wid && s > before) ? s : before;
-  }
+  const uint32_t s = lane < nw ? scratch[lane] : 0u;
+  const uint32_t sx = wave_shr1(wave_incl_scan(s, 0u, mx), 0u);
+  const uint32_t waves_before = (uint32_t)__builtin_amdgcn_readlane((int)sx, wid);
   __syncthreads();
-  return before;
+  return mx(waves_before, wave_shr1(incl, 0u));
 }
 
 // Block-wide exclusive max (identity 0) for blockDim.x <= 1024; `scratch`

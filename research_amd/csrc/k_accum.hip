@@ -963,38 +963,32 @@ __global__ __launch_bounds__(kTileBlock) void k_num_tiles(
 // tile is written with coalesced stores.  Tiles with more outputs than
 // slots are finished in rank ranges.  Segment bounds come from the
 // tile-start table (granularity 2^logT columns).
-// Shape of a rank-tile workgroup of BS threads: A entries staged per chunk,
-// products per owner map, widest tile (2^MAXLOG columns) and its bitmap
-// words.  Smaller workgroups take a proportional share of the CU's LDS, so
-// several rows are in flight per CU and one row's LDS phases overlap
-// another's gathers.
+// Shape of a rank-tile workgroup of BS threads: A entries staged per chunk
+// (E per thread), products per owner map (U per thread), workgroups per
+// CU, widest tile (2^MAXLOG columns) and its bitmap words.  1024 and 512
+// threads take a whole CU's LDS (the 512-thread shape has twice the
+// registers per thread, so it holds 16 products per thread and two A
+// entries); 256 threads take a quarter, so four rows are in flight per CU.
 template <int BS>
 struct RankShape {
-  static constexpr uint32_t A = BS;
-  static constexpr uint32_t ROUND = 8 * BS;
-  static constexpr uint32_t MAXLOG = BS >= 1024 ? 17 : (BS >= 512 ? 16 : 15);
+  static constexpr uint32_t E = BS == 512 ? 2 : 1;
+  static constexpr uint32_t U = BS == 512 ? 16 : 8;
+  static constexpr uint32_t PER_CU = BS == 256 ? 4 : 1;
+  static constexpr uint32_t A = E * BS;
+  static constexpr uint32_t ROUND = U * BS;
+  static constexpr uint32_t MAXLOG = PER_CU == 1 ? 17 : 15;
   static constexpr uint32_t WORDS = (1u << MAXLOG) / 32;
   static size_t lds_fixed() {
     return 256 + (size_t)A * (4 + 4 + 8) + (size_t)ROUND * 2 + (size_t)WORDS * 8;
   }
 };
 
-// OR the column bits of up to 8 products into the tile bitmap.
-__device__ __forceinline__ void mark8(const uint32_t (&col)[8], uint64_t wbase, uint32_t *bits) {
-#pragma unroll
-  for (int u = 0; u < 8; ++u)
-    if (col[u] != kEmpty) {
-      const uint32_t x = (uint32_t)(col[u] - wbase);
-      atomicOr(&bits[x >> 5], 1u << (x & 31));
-    }
-}
-
 // One workgroup per long row (rows taken from an atomic counter).  A
 // chunk's products are flattened (A entry i owns positions [ex_i,
-// ex_i + len_i)) and enumerated in rounds of 8192: an LDS owner map, built
-// by marking each segment's first position and a workgroup max-scan, gives
-// every position its A entry with one LDS read, and lane-strided positions
-// keep the B gathers coalesced.
+// ex_i + len_i)) and enumerated in rounds of U * BS: an LDS owner map,
+// built by marking each segment's first position and a workgroup
+// max-scan, gives every position its A entry with one LDS read, and
+// lane-strided positions keep the B gathers coalesced.
 // Entries per B row of the compact tile-start table of the widest tile.
 constexpr uint32_t kWideStarts = 16;
 
@@ -1004,23 +998,25 @@ __global__ __launch_bounds__(BS) void k_num_rank(
     uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1,
     const uint32_t *__restrict__ tsw, const uint32_t *__restrict__ bbase, uint32_t cap,
     uint32_t pcap, KOut<T> o) {
+  using S = RankShape<BS>;
+  constexpr uint32_t E = S::E, U = S::U, V = U / 8;
   extern __shared__ __align__(16) uint8_t smem[];
   uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
   uint32_t *sk = reinterpret_cast<uint32_t *>(smem + 256);
-  uint32_t *sbase = sk + RankShape<BS>::A;  // B position of product p is sbase[owner] + p
-  double *sav = reinterpret_cast<double *>(sbase + RankShape<BS>::A);
-  uint16_t *own = reinterpret_cast<uint16_t *>(sav + RankShape<BS>::A);
-  uint32_t *bits = reinterpret_cast<uint32_t *>(own + RankShape<BS>::ROUND);
-  uint32_t *wpre = bits + RankShape<BS>::WORDS;
-  double *acc = reinterpret_cast<double *>(wpre + RankShape<BS>::WORDS);
+  uint32_t *sbase = sk + S::A;  // B position of product p is sbase[owner] + p
+  double *sav = reinterpret_cast<double *>(sbase + S::A);
+  uint16_t *own = reinterpret_cast<uint16_t *>(sav + S::A);
+  uint32_t *bits = reinterpret_cast<uint32_t *>(own + S::ROUND);
+  uint32_t *wpre = bits + S::WORDS;
+  double *acc = reinterpret_cast<double *>(wpre + S::WORDS);
   uint32_t *cmap = reinterpret_cast<uint32_t *>(acc + cap);
   typedef unsigned short us8 __attribute__((ext_vector_type(8)));
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < RankShape<BS>::WORDS; i += BS) bits[i] = 0;
+  for (uint32_t i = tid; i < S::WORDS; i += BS) bits[i] = 0;
   for (uint32_t i = tid; i < cap; i += BS) acc[i] = 0.0;
-  // owner entries are thread ids (< BS) from the start, so the unmasked
+  // owner entries are A entry ids (< S::A) from the start, so the unmasked
   // reads past a round's end stay inside sbase
-  for (uint32_t i = tid; i < RankShape<BS>::ROUND; i += BS) own[i] = 0;
+  for (uint32_t i = tid; i < S::ROUND; i += BS) own[i] = 0;
   __syncthreads();
   for (;;) {
     if (tid == 0) hdr[0] = atomicAdd(counter, 1u);
@@ -1041,12 +1037,12 @@ __global__ __launch_bounds__(BS) void k_num_rank(
     if (o.nnz_hint) est = o.nnz_hint[r];
     const uint64_t prod = a.products[r];
     uint32_t lw = logT;
-    while (lw < RankShape<BS>::MAXLOG && (uint64_t)(span >> lw) >= 1 &&
+    while (lw < S::MAXLOG && (uint64_t)(span >> lw) >= 1 &&
            est * (1ull << (lw + 1)) <= (uint64_t)cap * 3 / 4 * span &&
            (pcap == 0 || prod * (1ull << (lw + 1)) <= (uint64_t)pcap * span))
       ++lw;
     const uint32_t step = lw - logT;
-    const bool one_chunk = nA <= RankShape<BS>::A;
+    const bool one_chunk = nA <= S::A;
     const uint64_t off = o.off[r];
     // estimated capacity (estimation workflow): a row whose outputs exceed
     // it stops before writing past its region and joins the overflow list
@@ -1060,93 +1056,129 @@ __global__ __launch_bounds__(BS) void k_num_rank(
     const uint32_t nwords = 1u << (lw - 5);
     const uint32_t wpt = (nwords + BS - 1) / BS;
     const uint32_t b0 = min(nwords, tid * wpt), b1 = min(nwords, b0 + wpt);
-    uint32_t my_ex = 0, my_len = 0;  // this thread's A entry segment
+    // segments of this thread's A entries tid * E + j
+    uint32_t my_ex[E], my_len[E];
 
-    // stage chunk c0.. of the A row and the segment bounds of tile t
+    // stage chunk c0.. of the A row
     auto stage = [&](uint32_t c0, uint32_t nc) {
-      if (tid < nc) {
-        sk[tid] = a.a_ci[c0 + tid];
-        sav[tid] = (double)a.a_v[c0 + tid];
+      for (uint32_t j = tid; j < nc; j += BS) {
+        sk[j] = a.a_ci[c0 + j];
+        sav[j] = (double)a.a_v[c0 + j];
       }
       __syncthreads();
     };
-    // Tile starts of this thread's A entry.  Single-chunk rows walk their
-    // tiles in order, so a tile's end is the next tile's start and the end
-    // after it is loaded one tile ahead.  Rows at the widest tile read the
-    // compact table (all of a B row's starts in one 64-byte slot, so the
-    // line a row's tiles share stays cached between them).
-    const bool wide = tsw != nullptr && lw == RankShape<BS>::MAXLOG;
+    // Tile starts of an A entry.  Single-chunk rows walk their tiles in
+    // order, so a tile's end is the next tile's start and the end after it
+    // is loaded one tile ahead.  Rows at the widest tile read the compact
+    // table (all of a B row's starts in one 64-byte slot, so the line a
+    // row's tiles share stays cached between them).
+    const bool wide = tsw != nullptr && lw == S::MAXLOG;
     auto tsv = [&](uint32_t k, uint32_t tt) {
       if (wide) return tsw[(uint64_t)k * kWideStarts + min(kWideStarts - 1, tt)];
       return ts[(uint64_t)k * T1 + min(T1 - 1, tt << step)];
     };
-    uint32_t ts_cur = 0, ts_end = 0;
+    uint32_t ts_cur[E], ts_end[E];
+    // segment bounds of tile t for the staged chunk of nc entries
     auto bounds = [&](uint32_t t, uint32_t nc, uint32_t &P) {
-      uint32_t len = 0, st = 0;
-      if (tid < nc) {
-        if (one_chunk) {
-          st = ts_cur;
-          len = ts_end - ts_cur;
-          ts_cur = ts_end;
-          if (t + 1 <= t_end) ts_end = tsv(sk[tid], t + 2);
-        } else {
-          st = tsv(sk[tid], t);
-          len = tsv(sk[tid], t + 1) - st;
+      uint32_t st[E], len[E], sum = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < E; ++j) {
+        const uint32_t e = tid * E + j;
+        st[j] = 0;
+        len[j] = 0;
+        if (e < nc) {
+          if (one_chunk) {
+            st[j] = ts_cur[j];
+            len[j] = ts_end[j] - ts_cur[j];
+            ts_cur[j] = ts_end[j];
+            if (t + 1 <= t_end) ts_end[j] = tsv(sk[e], t + 2);
+          } else {
+            st[j] = tsv(sk[e], t);
+            len[j] = tsv(sk[e], t + 1) - st[j];
+          }
         }
+        sum += len[j];
       }
-      const uint32_t ex = block_excl_sum(len, hdr + 8, &P);
-      my_ex = ex;
-      my_len = len;
-      if (tid < nc) sbase[tid] = st - ex;
+      uint32_t ex = block_excl_sum(sum, hdr + 8, &P);
+#pragma unroll
+      for (uint32_t j = 0; j < E; ++j) {
+        my_ex[j] = ex;
+        my_len[j] = len[j];
+        if (tid * E + j < nc) sbase[tid * E + j] = st[j] - ex;
+        ex += len[j];
+      }
       __syncthreads();
     };
-    // owner map of positions [p0, p0 + n)
+    // owner map of positions [p0, p0 + n): thread tid holds positions
+    // [U tid, U tid + U)
     auto build_map = [&](uint32_t p0, uint32_t n) {
       __syncthreads();  // the previous map is no longer read
-      if (8 * tid < n) *reinterpret_cast<us8 *>(own + 8 * tid) = us8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (uint32_t v = 0; v < V; ++v)
+        if (U * tid + 8 * v < n)
+          *reinterpret_cast<us8 *>(own + U * tid + 8 * v) = us8{0, 0, 0, 0, 0, 0, 0, 0};
       __syncthreads();
-      if (my_len > 0 && my_ex < p0 + n && my_ex + my_len > p0)
-        own[my_ex > p0 ? my_ex - p0 : 0] = (uint16_t)tid;
+#pragma unroll
+      for (uint32_t j = 0; j < E; ++j)
+        if (my_len[j] > 0 && my_ex[j] < p0 + n && my_ex[j] + my_len[j] > p0)
+          own[my_ex[j] > p0 ? my_ex[j] - p0 : 0] = (uint16_t)(tid * E + j);
       __syncthreads();
-      us8 v = (8 * tid < n) ? *reinterpret_cast<const us8 *>(own + 8 * tid) : us8{0, 0, 0, 0, 0, 0, 0, 0};
+      us8 vv[V];
       uint32_t run = 0;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        run = max(run, (uint32_t)v[j]);
-        v[j] = (unsigned short)run;
+      for (uint32_t v = 0; v < V; ++v) {
+        vv[v] = (U * tid + 8 * v < n) ? *reinterpret_cast<const us8 *>(own + U * tid + 8 * v)
+                                      : us8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          run = max(run, (uint32_t)vv[v][k]);
+          vv[v][k] = (unsigned short)run;
+        }
       }
       const uint32_t pre = block_excl_max(run, hdr + 32);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = (unsigned short)max(pre, (uint32_t)v[j]);
-      if (8 * tid < n) *reinterpret_cast<us8 *>(own + 8 * tid) = v;
+      for (uint32_t v = 0; v < V; ++v) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) vv[v][k] = (unsigned short)max(pre, (uint32_t)vv[v][k]);
+        if (U * tid + 8 * v < n) *reinterpret_cast<us8 *>(own + U * tid + 8 * v) = vv[v];
+      }
       __syncthreads();
     };
     // products p0 + tid + BS u of the current map: B entry q[u] (kEmpty
     // past n), A entry w[u].  All owner reads are issued before any sbase
     // read, with no lane branches.
-    auto locate = [&](uint32_t p0, uint32_t n, uint32_t (&q)[8], uint32_t (&w)[8]) {
+    auto locate = [&](uint32_t p0, uint32_t n, uint32_t (&q)[U], uint32_t (&w)[U]) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) w[u] = own[tid + (uint32_t)u * BS];
+      for (uint32_t u = 0; u < U; ++u) w[u] = own[tid + u * BS];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const uint32_t i = tid + (uint32_t)u * BS;
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t i = tid + u * BS;
         const uint32_t sb = sbase[w[u]];
         q[u] = i < n ? sb + p0 + i : kEmpty;
       }
     };
+    // the column bits of the products into the tile bitmap
+    auto mark = [&](const uint32_t (&col)[U], uint64_t wbase) {
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u)
+        if (col[u] != kEmpty) {
+          const uint32_t x = (uint32_t)(col[u] - wbase);
+          atomicOr(&bits[x >> 5], 1u << (x & 31));
+        }
+    };
     // products (column, value) summed into ranks [r0, r1): all bitmap and
     // prefix reads first, then the adds
-    auto accumulate = [&](const uint32_t (&col)[8], const double (&pv)[8], uint64_t wbase,
+    auto accumulate = [&](const uint32_t (&col)[U], const double (&pv)[U], uint64_t wbase,
                           uint32_t r0, uint32_t r1) {
-      uint32_t wp[8], bw[8];
+      uint32_t wp[U], bw[U];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (uint32_t u = 0; u < U; ++u) {
         const uint32_t wd = col[u] != kEmpty ? (uint32_t)(col[u] - wbase) >> 5 : 0u;
         wp[u] = wpre[wd];
         bw[u] = bits[wd];
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (uint32_t u = 0; u < U; ++u) {
         const uint32_t x = (uint32_t)(col[u] - wbase);
         const uint32_t rk = wp[u] + __popc(bw[u] & ((1u << (x & 31)) - 1u));
         if (col[u] != kEmpty && rk >= r0 && rk < r1) {
@@ -1157,9 +1189,13 @@ __global__ __launch_bounds__(BS) void k_num_rank(
     };
     if (one_chunk) {
       stage(a0, nA);
-      if (tid < nA) {
-        ts_cur = tsv(sk[tid], t_begin);
-        ts_end = tsv(sk[tid], t_begin + 1);
+#pragma unroll
+      for (uint32_t j = 0; j < E; ++j) {
+        ts_cur[j] = ts_end[j] = 0;
+        if (tid * E + j < nA) {
+          ts_cur[j] = tsv(sk[tid * E + j], t_begin);
+          ts_end[j] = tsv(sk[tid * E + j], t_begin + 1);
+        }
       }
     }
     for (uint32_t t = t_begin; t <= t_end; ++t) {
@@ -1167,10 +1203,15 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       // ---- pass 1: mark the output columns
       uint32_t any = 0, Plast = 0;
       bool keep = false;
-      uint32_t kc[8];  // kept columns and products of round 0 (single-chunk rows)
-      double kv[8];
-      for (uint32_t c0 = a0; c0 < a1; c0 += RankShape<BS>::A) {
-        const uint32_t nc = min(RankShape<BS>::A, a1 - c0);
+      uint32_t kc[U];  // kept columns and products of round 0 (single-chunk rows)
+      double kv[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {  // defined each tile: nothing carried over
+        kc[u] = kEmpty;
+        kv[u] = 0.0;
+      }
+      for (uint32_t c0 = a0; c0 < a1; c0 += S::A) {
+        const uint32_t nc = min(S::A, a1 - c0);
         if (!one_chunk) stage(c0, nc);
         uint32_t P;
         bounds(t, nc, P);
@@ -1180,21 +1221,21 @@ __global__ __launch_bounds__(BS) void k_num_rank(
         // single-chunk rows take the rounds last to first, so round 0 (the
         // only full one when there are two) ends pass 1 with its columns and
         // products in registers, and pass 2 sums it without a gather
-        const uint32_t nr = (P + RankShape<BS>::ROUND - 1) / RankShape<BS>::ROUND;
+        const uint32_t nr = (P + S::ROUND - 1) / S::ROUND;
         for (uint32_t ri = 0; ri < nr; ++ri) {
-          const uint32_t p0 = (one_chunk ? nr - 1 - ri : ri) * RankShape<BS>::ROUND;
-          const uint32_t n = min(RankShape<BS>::ROUND, P - p0);
+          const uint32_t p0 = (one_chunk ? nr - 1 - ri : ri) * S::ROUND;
+          const uint32_t n = min(S::ROUND, P - p0);
           build_map(p0, n);
-          uint32_t q[8], w[8];
+          uint32_t q[U], w[U];
           locate(p0, n, q, w);
 #pragma unroll
-          for (int u = 0; u < 8; ++u) kc[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
+          for (uint32_t u = 0; u < U; ++u) kc[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
           if (one_chunk && p0 == 0) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (uint32_t u = 0; u < U; ++u)
               kv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
           }
-          mark8(kc, wbase, bits);
+          mark(kc, wbase);
         }
         if (!one_chunk) __syncthreads();
       }
@@ -1221,12 +1262,12 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       // columns and values gathered, summed into ranks [r0, r1)
       auto round_sum = [&](uint32_t p0, uint32_t n, uint32_t r0, uint32_t r1) {
         build_map(p0, n);
-        uint32_t q[8], w[8];
+        uint32_t q[U], w[U];
         locate(p0, n, q, w);
-        uint32_t col[8];
-        double pv[8];
+        uint32_t col[U];
+        double pv[U];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (uint32_t u = 0; u < U; ++u) {
           col[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
           pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
         }
@@ -1234,15 +1275,15 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       };
       // every round of the tile summed into ranks [r0, r1)
       auto sum_rounds = [&](uint32_t r0, uint32_t r1) {
-        for (uint32_t c0 = a0; c0 < a1; c0 += RankShape<BS>::A) {
-          const uint32_t nc = min(RankShape<BS>::A, a1 - c0);
+        for (uint32_t c0 = a0; c0 < a1; c0 += S::A) {
+          const uint32_t nc = min(S::A, a1 - c0);
           uint32_t P = Plast;
           if (!one_chunk) {
             stage(c0, nc);
             bounds(t, nc, P);
           }
-          for (uint32_t p0 = 0; p0 < P; p0 += RankShape<BS>::ROUND)
-            round_sum(p0, min(RankShape<BS>::ROUND, P - p0), r0, r1);
+          for (uint32_t p0 = 0; p0 < P; p0 += S::ROUND)
+            round_sum(p0, min(S::ROUND, P - p0), r0, r1);
           if (!one_chunk) __syncthreads();
         }
       };
@@ -1261,8 +1302,8 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       const uint32_t r1a = min(m, cap);
       if (keep) {
         accumulate(kc, kv, wbase, 0, r1a);
-        for (uint32_t p0 = RankShape<BS>::ROUND; p0 < Plast; p0 += RankShape<BS>::ROUND)
-          round_sum(p0, min(RankShape<BS>::ROUND, Plast - p0), 0, r1a);
+        for (uint32_t p0 = S::ROUND; p0 < Plast; p0 += S::ROUND)
+          round_sum(p0, min(S::ROUND, Plast - p0), 0, r1a);
       } else {
         sum_rounds(0, r1a);
       }
@@ -1305,11 +1346,13 @@ int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint
   uint32_t *counter = (uint32_t *)scratch;
   OC_TRY(zero_u32(counter, 1, s));
   if (bs != 256 && bs != 512) bs = 1024;
-  const uint32_t per_cu = 1024 / bs;  // workgroups resident per CU
-  const size_t budget = std::min<size_t>(t.lds_max_block, t.lds_per_cu / per_cu) & ~255ull;
-  const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu) * per_cu);
 #define GO(T, B)                                                                    \
   do {                                                                              \
+    const uint32_t per_cu = RankShape<B>::PER_CU;                                   \
+    const size_t budget =                                                           \
+        std::min<size_t>(t.lds_max_block, t.lds_per_cu / per_cu) & ~255ull;         \
+    const uint32_t grid =                                                           \
+        std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu) * per_cu);        \
     const size_t fixed = RankShape<B>::lds_fixed();                                 \
     if (fixed + 64 * 12 > budget) {                                                 \
       set_error("rank-tile kernel does not fit the LDS of this device");            \
