@@ -965,22 +965,23 @@ __global__ __launch_bounds__(kTileBlock) void k_num_tiles(
 // tile-start table (granularity 2^logT columns).
 // Shape of a rank-tile workgroup of BS threads: A entries staged per chunk
 // (E per thread), products per owner map (U per thread), workgroups per
-// CU, widest tile (2^MAXLOG columns) and its bitmap words.  1024 and 512
-// threads take a whole CU's LDS (the 512-thread shape has twice the
-// registers per thread, so it holds 16 products per thread and two A
-// entries); 256 threads take a quarter, so four rows are in flight per CU.
+// CU, widest tile (2^MAXLOG columns) and its bitmap words.  1024 threads
+// take a whole CU's LDS; 512 threads (two A entries each) take half and
+// 256 a quarter, so two or four rows are in flight per CU and one row's
+// LDS phases overlap another's gathers.
 template <int BS>
 struct RankShape {
   static constexpr uint32_t E = BS == 512 ? 2 : 1;
-  static constexpr uint32_t U = BS == 512 ? 16 : 8;
-  static constexpr uint32_t PER_CU = BS == 256 ? 4 : 1;
+  static constexpr uint32_t U = 8;
+  static constexpr uint32_t PER_CU = 1024 / BS;
   static constexpr uint32_t A = E * BS;
   static constexpr uint32_t ROUND = U * BS;
-  static constexpr uint32_t MAXLOG = PER_CU == 1 ? 17 : 15;
+  static constexpr uint32_t MAXLOG = BS >= 1024 ? 17 : (BS >= 512 ? 16 : 15);
   static constexpr uint32_t WORDS = (1u << MAXLOG) / 32;
   static size_t lds_fixed() {
     return 256 + (size_t)A * (4 + 4 + 8) + (size_t)ROUND * 2 + (size_t)WORDS * 8;
   }
+  static_assert(A <= 1024, "owner-map entries hold 10-bit A entry ids");
 };
 
 // One workgroup per long row (rows taken from an atomic counter).  A
@@ -991,9 +992,12 @@ struct RankShape {
 // lane-strided positions keep the B gathers coalesced.
 // Entries per B row of the compact tile-start table of the widest tile.
 constexpr uint32_t kWideStarts = 16;
+// Owner-map entries: A entry id in the low bits, map tag above it.
+constexpr uint32_t kRankIdBits = 10, kRankIdMask = (1u << kRankIdBits) - 1;
+constexpr uint32_t kRankTags = 1u << (16 - kRankIdBits);
 
 template <typename T, int BS>
-__global__ __launch_bounds__(BS) void k_num_rank(
+__global__ __launch_bounds__(BS, 4) void k_num_rank(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
     uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1,
     const uint32_t *__restrict__ tsw, const uint32_t *__restrict__ bbase, uint32_t cap,
@@ -1014,9 +1018,10 @@ __global__ __launch_bounds__(BS) void k_num_rank(
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < S::WORDS; i += BS) bits[i] = 0;
   for (uint32_t i = tid; i < cap; i += BS) acc[i] = 0.0;
-  // owner entries are A entry ids (< S::A) from the start, so the unmasked
-  // reads past a round's end stay inside sbase
+  // owner entries hold A entry ids (< S::A) from the start, so the
+  // unmasked reads past a round's end stay inside sbase
   for (uint32_t i = tid; i < S::ROUND; i += BS) own[i] = 0;
+  uint32_t tag = 0;  // tag of the current owner map
   __syncthreads();
   for (;;) {
     if (tid == 0) hdr[0] = atomicAdd(counter, 1u);
@@ -1109,19 +1114,32 @@ __global__ __launch_bounds__(BS) void k_num_rank(
       }
       __syncthreads();
     };
-    // owner map of positions [p0, p0 + n): thread tid holds positions
-    // [U tid, U tid + U)
+    // Owner map of positions [p0, p0 + n).  Thread tid holds positions
+    // [U tid, U tid + U), so a wave holds one block of U * 64.  A segment
+    // marks its first position in the map and the first position of every
+    // later block it covers; every block's first position is then marked
+    // and a wave-level max-scan completes the map without crossing waves.
+    // Marks carry the map's tag above the entry id, so the max picks this
+    // map's marks over stale ones and nothing is cleared between maps (the
+    // whole map is cleared when the tag wraps).
     auto build_map = [&](uint32_t p0, uint32_t n) {
+      constexpr uint32_t BLK = U * kWave;
       __syncthreads();  // the previous map is no longer read
-#pragma unroll
-      for (uint32_t v = 0; v < V; ++v)
-        if (U * tid + 8 * v < n)
-          *reinterpret_cast<us8 *>(own + U * tid + 8 * v) = us8{0, 0, 0, 0, 0, 0, 0, 0};
-      __syncthreads();
+      if (++tag == kRankTags) {
+        for (uint32_t i = tid; i < S::ROUND; i += BS) own[i] = 0;
+        tag = 1;
+        __syncthreads();
+      }
+      const uint32_t tg = tag << kRankIdBits;
 #pragma unroll
       for (uint32_t j = 0; j < E; ++j)
-        if (my_len[j] > 0 && my_ex[j] < p0 + n && my_ex[j] + my_len[j] > p0)
-          own[my_ex[j] > p0 ? my_ex[j] - p0 : 0] = (uint16_t)(tid * E + j);
+        if (my_len[j] > 0 && my_ex[j] < p0 + n && my_ex[j] + my_len[j] > p0) {
+          const uint32_t st = my_ex[j] > p0 ? my_ex[j] - p0 : 0u;
+          const uint32_t en = min(my_ex[j] + my_len[j] - p0, n);
+          const uint16_t mk = (uint16_t)(tg | (tid * E + j));
+          own[st] = mk;
+          for (uint32_t b = (st / BLK + 1) * BLK; b < en; b += BLK) own[b] = mk;
+        }
       __syncthreads();
       us8 vv[V];
       uint32_t run = 0;
@@ -1135,7 +1153,8 @@ __global__ __launch_bounds__(BS) void k_num_rank(
           vv[v][k] = (unsigned short)run;
         }
       }
-      const uint32_t pre = block_excl_max(run, hdr + 32);
+      auto mx = [](uint32_t x, uint32_t y) { return x > y ? x : y; };
+      const uint32_t pre = wave_shr1(wave_incl_scan(run, 0u, mx), 0u);
 #pragma unroll
       for (uint32_t v = 0; v < V; ++v) {
 #pragma unroll
@@ -1149,7 +1168,7 @@ __global__ __launch_bounds__(BS) void k_num_rank(
     // read, with no lane branches.
     auto locate = [&](uint32_t p0, uint32_t n, uint32_t (&q)[U], uint32_t (&w)[U]) {
 #pragma unroll
-      for (uint32_t u = 0; u < U; ++u) w[u] = own[tid + u * BS];
+      for (uint32_t u = 0; u < U; ++u) w[u] = own[tid + u * BS] & kRankIdMask;
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
         const uint32_t i = tid + u * BS;
