@@ -1240,20 +1240,25 @@ __global__ __launch_bounds__(BS, 4) void k_num_rank(
         // single-chunk rows take the rounds last to first, so round 0 (the
         // only full one when there are two) ends pass 1 with its columns and
         // products in registers, and pass 2 sums it without a gather
+        // the next round's owner map is built while this round's gathers
+        // are in flight (the map is read before them, the gathers are used
+        // after it)
         const uint32_t nr = (P + S::ROUND - 1) / S::ROUND;
+        auto rp0 = [&](uint32_t ri) { return (one_chunk ? nr - 1 - ri : ri) * S::ROUND; };
+        if (nr) build_map(rp0(0), min(S::ROUND, P - rp0(0)));
         for (uint32_t ri = 0; ri < nr; ++ri) {
-          const uint32_t p0 = (one_chunk ? nr - 1 - ri : ri) * S::ROUND;
+          const uint32_t p0 = rp0(ri);
           const uint32_t n = min(S::ROUND, P - p0);
-          build_map(p0, n);
           uint32_t q[U], w[U];
           locate(p0, n, q, w);
 #pragma unroll
           for (uint32_t u = 0; u < U; ++u) kc[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
-          if (one_chunk && p0 == 0) {
+          if (one_chunk && p0 == 0) {  // the last round: nothing to build
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
               kv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
           }
+          if (ri + 1 < nr) build_map(rp0(ri + 1), min(S::ROUND, P - rp0(ri + 1)));
           mark(kc, wbase);
         }
         if (!one_chunk) __syncthreads();
@@ -1277,20 +1282,28 @@ __global__ __launch_bounds__(BS, 4) void k_num_rank(
         break;
       }
       // ---- pass 2: sums by rank (in rank ranges when m > cap)
-      // products [p0, p0 + n) of the current chunk: owner map rebuilt,
-      // columns and values gathered, summed into ranks [r0, r1)
-      auto round_sum = [&](uint32_t p0, uint32_t n, uint32_t r0, uint32_t r1) {
-        build_map(p0, n);
-        uint32_t q[U], w[U];
-        locate(p0, n, q, w);
-        uint32_t col[U];
-        double pv[U];
+      // products [first, P) of the current chunk, round by round: owner
+      // map, columns and values gathered, summed into ranks [r0, r1); the
+      // next round's map is built while this round's gathers are in flight
+      auto sum_from = [&](uint32_t first, uint32_t P, uint32_t r0, uint32_t r1) {
+        if (first >= P) return;
+        build_map(first, min(S::ROUND, P - first));
+        for (uint32_t p0 = first; p0 < P; p0 += S::ROUND) {
+          uint32_t q[U], w[U];
+          locate(p0, min(S::ROUND, P - p0), q, w);
+          uint32_t col[U];
+          T bv[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-          col[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
-          pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)a.b_v[q[u]] : 0.0;
+          for (uint32_t u = 0; u < U; ++u) {
+            col[u] = q[u] != kEmpty ? a.b_ci[q[u]] : kEmpty;
+            bv[u] = q[u] != kEmpty ? a.b_v[q[u]] : T(0);
+          }
+          if (p0 + S::ROUND < P) build_map(p0 + S::ROUND, min(S::ROUND, P - p0 - S::ROUND));
+          double pv[U];
+#pragma unroll
+          for (uint32_t u = 0; u < U; ++u) pv[u] = q[u] != kEmpty ? sav[w[u]] * (double)bv[u] : 0.0;
+          accumulate(col, pv, wbase, r0, r1);
         }
-        accumulate(col, pv, wbase, r0, r1);
       };
       // every round of the tile summed into ranks [r0, r1)
       auto sum_rounds = [&](uint32_t r0, uint32_t r1) {
@@ -1301,8 +1314,7 @@ __global__ __launch_bounds__(BS, 4) void k_num_rank(
             stage(c0, nc);
             bounds(t, nc, P);
           }
-          for (uint32_t p0 = 0; p0 < P; p0 += S::ROUND)
-            round_sum(p0, min(S::ROUND, P - p0), r0, r1);
+          sum_from(0, P, r0, r1);
           if (!one_chunk) __syncthreads();
         }
       };
@@ -1321,8 +1333,7 @@ __global__ __launch_bounds__(BS, 4) void k_num_rank(
       const uint32_t r1a = min(m, cap);
       if (keep) {
         accumulate(kc, kv, wbase, 0, r1a);
-        for (uint32_t p0 = S::ROUND; p0 < Plast; p0 += S::ROUND)
-          round_sum(p0, min(S::ROUND, Plast - p0), 0, r1a);
+        sum_from(S::ROUND, Plast, 0, r1a);
       } else {
         sum_rounds(0, r1a);
       }
