@@ -192,8 +192,8 @@ def main():
                     help="only run three 1 GiB device-to-device copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(row panels of A would shard across GPUs; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(row panels of A would be the shards, one per GPU).", file=sys.stderr)
         sys.exit(2)
 
     import research_amd as ra
