@@ -166,6 +166,46 @@ __device__ __forceinline__ void seg_vals(uint32_t s, uint32_t e, int lig, int G,
   for (; q < e; q += step) f(b_ci[q], av * (double)b_v[q]);
 }
 
+// seg_cols / seg_vals with the U entries of a full step handed over at
+// once (fb), so the callee can issue their LDS operations together; the
+// tail goes one entry at a time (f1).
+template <int U, typename FB, typename F1>
+__device__ __forceinline__ void seg_cols_batch(uint32_t s, uint32_t e, int lig, int G,
+                                               const uint32_t *__restrict__ b_ci, FB &&fb,
+                                               F1 &&f1) {
+  uint32_t q = s + lig;
+  const uint32_t step = (uint32_t)G;
+  for (; q + (U - 1) * step < e; q += U * step) {
+    uint32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = b_ci[q + u * step];
+    fb(c);
+  }
+  for (; q < e; q += step) f1(b_ci[q]);
+}
+
+template <int U, typename T, typename FB, typename F1>
+__device__ __forceinline__ void seg_vals_batch(uint32_t s, uint32_t e, int lig, int G, double av,
+                                               const uint32_t *__restrict__ b_ci,
+                                               const T *__restrict__ b_v, FB &&fb, F1 &&f1) {
+  uint32_t q = s + lig;
+  const uint32_t step = (uint32_t)G;
+  for (; q + (U - 1) * step < e; q += U * step) {
+    uint32_t c[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      c[u] = b_ci[q + u * step];
+      v[u] = b_v[q + u * step];
+    }
+    double pv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) pv[u] = av * (double)v[u];
+    fb(c, pv);
+  }
+  for (; q < e; q += step) f1(b_ci[q], av * (double)b_v[q]);
+}
+
 // =================================================================== ESC
 // One product per lane of an N-lane group; the group ranks its N
 // (col, product) pairs by compare-counting and sums equal columns in
@@ -331,25 +371,41 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
   const bool track = limit < S;
   uint32_t mine = 0;
   bool ovf = false;
-  auto insert = [&](uint32_t c) {
-    if (ovf) return;
-    uint32_t slot = hash_slot(c, S);
-    uint32_t probes = 0;
+  // linear probing from `slot` (`probes` slots already tried), each probe
+  // a compare-and-swap that either claims an empty slot or finds the key
+  auto probe = [&](uint32_t c, uint32_t slot, uint32_t probes) {
     for (; probes < S; ++probes) {
-      const uint32_t v = tab[slot];
-      if (v == c) break;
-      if (v == kEmpty) {
-        const uint32_t old = atomicCAS(&tab[slot], kEmpty, c);
-        if (old == kEmpty) {
-          ++mine;
-          if (track) atomicAdd(&hdr[0], 1u);
-          break;
-        }
-        if (old == c) break;
+      const uint32_t old = atomicCAS(&tab[slot], kEmpty, c);
+      if (old == kEmpty) {
+        ++mine;
+        if (track) atomicAdd(&hdr[0], 1u);
+        return;
       }
+      if (old == c) return;
       if (++slot == S) slot = 0;
     }
-    if (probes == S) ovf = true;
+    ovf = true;
+  };
+  auto insert = [&](uint32_t c) {
+    if (!ovf) probe(c, hash_slot(c, S), 0);
+  };
+  // eight keys: first probes issued together, collisions probed on
+  auto insert8 = [&](const uint32_t (&c)[8]) {
+    if (ovf) return;
+    uint32_t sl[8], old[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sl[u] = hash_slot(c[u], S);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) old[u] = atomicCAS(&tab[sl[u]], kEmpty, c[u]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (old[u] == kEmpty) {
+        ++mine;
+        if (track) atomicAdd(&hdr[0], 1u);
+      } else if (old[u] != c[u] && !ovf) {
+        probe(c[u], sl[u] + 1 == S ? 0u : sl[u] + 1, 1);
+      }
+    }
   };
   uint32_t heavy;
   const uint32_t nheavy =
@@ -362,7 +418,7 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
     const uint32_t k = a.a_ci[j];
     const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
     if (nheavy && be - bs > heavy) continue;  // listed: swept below
-    seg_cols<8>(bs, be, lig, G, a.b_ci, insert);
+    seg_cols_batch<8>(bs, be, lig, G, a.b_ci, insert8, insert);
   }
   __syncthreads();
   for (uint32_t h = 0; h < nheavy && !ovf; ++h) {
@@ -371,7 +427,7 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
       break;
     }
     const uint32_t k = a.a_ci[hdr[8 + h]];
-    seg_cols<8>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, insert);
+    seg_cols_batch<8>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, insert8, insert);
   }
   if (ovf) hdr[1] = 1;
   atomicAdd(&hdr[2], mine);
@@ -490,28 +546,41 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
   uint32_t limit = S;
   if (EST) limit = min(L.nmax - 1, o.cap[r]);
   bool ovf = false;
-  auto insert = [&](uint32_t c, double pv) {
-    if (ovf) return;
-    uint32_t slot = hash_slot(c, S);
-    uint32_t probes = 0;
+  // linear probing from `slot` (`probes` slots already tried), each probe
+  // a compare-and-swap that either claims an empty slot or finds the key;
+  // the product is added to the slot found
+  auto probe = [&](uint32_t c, double pv, uint32_t slot, uint32_t probes) {
     for (; probes < S; ++probes) {
-      const uint32_t v = keys[slot];
-      if (v == c) break;
-      if (v == kEmpty) {
-        const uint32_t old = atomicCAS(&keys[slot], kEmpty, c);
-        if (old == kEmpty) {
-          if (EST) atomicAdd(&hdr[0], 1u);
-          break;
-        }
-        if (old == c) break;
+      const uint32_t old = atomicCAS(&keys[slot], kEmpty, c);
+      if (old == kEmpty || old == c) {
+        if (EST && old == kEmpty) atomicAdd(&hdr[0], 1u);
+        lds_add_f64(&vals[slot], pv);
+        return;
       }
       if (++slot == S) slot = 0;
     }
-    if (probes == S) {
-      ovf = true;
-      return;
+    ovf = true;
+  };
+  auto insert = [&](uint32_t c, double pv) {
+    if (!ovf) probe(c, pv, hash_slot(c, S), 0);
+  };
+  // four products: first probes issued together, collisions probed on
+  auto insert4 = [&](const uint32_t (&c)[4], const double (&pv)[4]) {
+    if (ovf) return;
+    uint32_t sl[4], old[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sl[u] = hash_slot(c[u], S);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) old[u] = atomicCAS(&keys[sl[u]], kEmpty, c[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (old[u] == kEmpty || old[u] == c[u]) {
+        if (EST && old[u] == kEmpty) atomicAdd(&hdr[0], 1u);
+        lds_add_f64(&vals[sl[u]], pv[u]);
+      } else if (!ovf) {
+        probe(c[u], pv[u], sl[u] + 1 == S ? 0u : sl[u] + 1, 1);
+      }
     }
-    lds_add_f64(&vals[slot], pv);
   };
   uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
   const uint32_t nheavy =
@@ -524,7 +593,7 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
     const uint32_t k = a.a_ci[j];
     const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
     if (nheavy && be - bs > heavy) continue;  // listed: swept below
-    seg_vals<4>(bs, be, lig, G, (double)a.a_v[j], a.b_ci, a.b_v, insert);
+    seg_vals_batch<4>(bs, be, lig, G, (double)a.a_v[j], a.b_ci, a.b_v, insert4, insert);
   }
   __syncthreads();
   for (uint32_t h = 0; h < nheavy && !ovf; ++h) {
@@ -534,8 +603,8 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
     }
     const uint32_t j = hdr[8 + h];
     const uint32_t k = a.a_ci[j];
-    seg_vals<4>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, (double)a.a_v[j], a.b_ci,
-                a.b_v, insert);
+    seg_vals_batch<4>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, (double)a.a_v[j], a.b_ci,
+                      a.b_v, insert4, insert);
   }
   if (ovf) hdr[1] = 1;
   __syncthreads();
