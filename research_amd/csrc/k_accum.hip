@@ -455,8 +455,11 @@ __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
     if (threadIdx.x == 0) nnz[r] = 0;
     return;
   }
-  const uint32_t nwords = ((hi - lo) >> 5) + 1;
-  for (uint32_t i = threadIdx.x; i < nwords; i += BS) bits[i] = 0;
+  // the bitmap is cleared and counted 128 bits at a time (the launch
+  // allocates whole 16-byte groups)
+  const uint32_t nwords = ((hi - lo) >> 5) + 1, n4 = (nwords + 3) / 4;
+  uint4 *bits4 = reinterpret_cast<uint4 *>(bits);
+  for (uint32_t i = threadIdx.x; i < n4; i += BS) bits4[i] = uint4{0, 0, 0, 0};
   if (threadIdx.x == 0) hdr[0] = hdr[4] = 0;
   __syncthreads();
   const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
@@ -480,7 +483,10 @@ __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
   }
   __syncthreads();
   uint32_t cnt = 0;
-  for (uint32_t i = threadIdx.x; i < nwords; i += BS) cnt += __popc(bits[i]);
+  for (uint32_t i = threadIdx.x; i < n4; i += BS) {
+    const uint4 v = bits4[i];
+    cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+  }
   cnt = group_sum(cnt, kWave);
   if (lane_id() == 0) atomicAdd(&hdr[0], cnt);
   __syncthreads();
@@ -1628,7 +1634,8 @@ int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
                      uint32_t nrows, uint32_t *nnz, hipStream_t s) {
   if (nrows == 0) return OC_OK;
   const Tables &t = tables();
-  const size_t lds = 256 + (size_t)(t.dense_sym[dense_bin] / 32 + 2) * 4;
+  // bitmap words of the widest span, rounded up to whole 16-byte groups
+  const size_t lds = 256 + (size_t)((t.dense_sym[dense_bin] / 32 + 2 + 3) & ~3u) * 4;
   KArgs<float> k = kargs<float>(a);
 #define GO(BS)                                                                 \
   do {                                                                         \
