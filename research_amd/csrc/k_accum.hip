@@ -1136,6 +1136,11 @@ __global__ __launch_bounds__(BS, 4) void k_num_rank(
     const uint32_t nwords = 1u << (lw - 5);
     const uint32_t wpt = (nwords + BS - 1) / BS;
     const uint32_t b0 = min(nwords, tid * wpt), b1 = min(nwords, b0 + wpt);
+    const bool quad = nwords == 4 * BS;  // words [4 tid, 4 tid + 4) per thread
+    auto clear_bits = [&] {
+      if (quad) reinterpret_cast<uint4 *>(bits)[tid] = uint4{0, 0, 0, 0};
+      else for (uint32_t i = b0; i < b1; ++i) bits[i] = 0;
+    };
     // segments of this thread's A entries tid * E + j
     uint32_t my_ex[E], my_len[E];
 
@@ -1340,18 +1345,27 @@ __global__ __launch_bounds__(BS, 4) void k_num_rank(
       }
       if (!any) continue;
       __syncthreads();
-      // ---- ranks: exclusive popcount prefix per bitmap word
-      uint32_t mycnt = 0;
-      for (uint32_t i = b0; i < b1; ++i) mycnt += __popc(bits[i]);
-      uint32_t m;
-      uint32_t run = block_excl_sum(mycnt, hdr + 8, &m);
-      for (uint32_t i = b0; i < b1; ++i) {
-        wpre[i] = run;
-        run += __popc(bits[i]);
+      // ---- ranks: exclusive popcount prefix per bitmap word (the widest
+      // tile gives each thread four aligned words: one 128-bit access)
+      uint32_t m, run;
+      if (quad) {
+        const uint4 w4 = reinterpret_cast<const uint4 *>(bits)[tid];
+        const uint32_t c0 = __popc(w4.x), c1 = __popc(w4.y), c2 = __popc(w4.z);
+        run = block_excl_sum(c0 + c1 + c2 + __popc(w4.w), hdr + 8, &m);
+        reinterpret_cast<uint4 *>(wpre)[tid] =
+            uint4{run, run + c0, run + c0 + c1, run + c0 + c1 + c2};
+      } else {
+        uint32_t mycnt = 0;
+        for (uint32_t i = b0; i < b1; ++i) mycnt += __popc(bits[i]);
+        run = block_excl_sum(mycnt, hdr + 8, &m);
+        for (uint32_t i = b0; i < b1; ++i) {
+          wpre[i] = run;
+          run += __popc(bits[i]);
+        }
       }
       __syncthreads();
       if (total + m > rcap) {  // m and total are uniform over the block
-        for (uint32_t i = b0; i < b1; ++i) bits[i] = 0;
+        clear_bits();
         __syncthreads();
         overflow = true;
         break;
@@ -1418,7 +1432,7 @@ __global__ __launch_bounds__(BS, 4) void k_num_rank(
         sum_rounds(r0, r1);
         write_ranks(r0, r1);
       }
-      for (uint32_t i = b0; i < b1; ++i) bits[i] = 0;
+      clear_bits();
       total += m;
       __syncthreads();
     }
