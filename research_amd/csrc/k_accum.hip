@@ -1062,9 +1062,11 @@ struct RankShape {
 // One workgroup per long row (rows taken from an atomic counter).  A
 // chunk's products are flattened (A entry i owns positions [ex_i,
 // ex_i + len_i)) and enumerated in rounds of U * BS: an LDS owner map,
-// built by marking each segment's first position and a workgroup
-// max-scan, gives every position its A entry with one LDS read, and
-// lane-strided positions keep the B gathers coalesced.
+// built by marking each segment's first position (and the first position
+// of every wave block it covers) and a wave-level max-scan, gives every
+// position its A entry with one LDS read, and lane-strided positions keep
+// the B gathers coalesced.  The next round's map is built while a round's
+// gathers are in flight.
 // Entries per B row of the compact tile-start table of the widest tile.
 constexpr uint32_t kWideStarts = 16;
 // Owner-map entries: A entry id in the low bits, map tag above it.
