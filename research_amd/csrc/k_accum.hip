@@ -129,8 +129,10 @@ __device__ __forceinline__ void lds_add_f64(double *p, double v) {
 
 // Walk one B row segment [s, e) with a group of G lanes (q = s + lig, step
 // G), U iterations at a time: the U column loads are issued before any of
-// them is used, so each lane keeps U gathers in flight.
-template <int U, typename F>
+// them is used, so each lane keeps U gathers in flight.  TAIL: the last
+// (at most U - 1) loads of a lane are issued together too, which pays on
+// the longest rows and costs the small-workgroup kernels a few percent.
+template <int U, bool TAIL = false, typename F>
 __device__ __forceinline__ void seg_cols(uint32_t s, uint32_t e, int lig, int G,
                                          const uint32_t *__restrict__ b_ci, F &&f) {
   uint32_t q = s + lig;
@@ -142,7 +144,16 @@ __device__ __forceinline__ void seg_cols(uint32_t s, uint32_t e, int lig, int G,
 #pragma unroll
     for (int u = 0; u < U; ++u) f(c[u]);
   }
-  for (; q < e; q += step) f(b_ci[q]);
+  if constexpr (TAIL) {
+    uint32_t c[U - 1];
+#pragma unroll
+    for (int u = 0; u < U - 1; ++u) c[u] = q + u * step < e ? b_ci[q + u * step] : 0u;
+#pragma unroll
+    for (int u = 0; u < U - 1; ++u)
+      if (q + u * step < e) f(c[u]);
+  } else {
+    for (; q < e; q += step) f(b_ci[q]);
+  }
 }
 
 // Same with the B values; f(col, av * b_value).
@@ -475,11 +486,11 @@ __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
     const uint32_t k = a.a_ci[j];
     const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
     if (nheavy && be - bs > heavy) continue;
-    seg_cols<8>(bs, be, lig, G, a.b_ci, mark);
+    seg_cols<8, BS == 1024>(bs, be, lig, G, a.b_ci, mark);
   }
   for (uint32_t h = 0; h < nheavy; ++h) {
     const uint32_t k = a.a_ci[hdr[32 + h]];
-    seg_cols<8>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, mark);
+    seg_cols<8, BS == 1024>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, mark);
   }
   __syncthreads();
   uint32_t cnt = 0;
