@@ -5,8 +5,10 @@ FP64 values uniform[-1,1) from seed 1, int32 indices).
 
 A "step" is one full C = A*A through libocean.so (analysis, HLL sampling,
 binning, symbolic, numeric, output allocation; the output is freed between
-steps).  A and B are resident in HBM before the timed region.  Prints one
-JSON line:
+steps).  A and B are resident in HBM before the timed region.  The product
+runs with the configuration the autotuner picked for this matrix
+(research_amd/tuned/rmat20.json, binning thresholds only, the same output;
+--config default gives the reference's defaults).  Prints one JSON line:
   value     GFLOP/s = 2 * intermediate products / step time
   roofline  dominant kernel family, its algorithmic bytes per launch over its
             average launch time (HIP events recorded on the launch stream by
@@ -188,6 +190,10 @@ def main():
                          "512 beside the default workload, 0 to skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config", default=None,
+                    help="JSON of configuration keys for the product (default for rmat20: the "
+                         "autotuner's research_amd/tuned/rmat20.json; 'default': the "
+                         "reference's defaults)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (counter calibration)")
     args = ap.parse_args()
@@ -233,7 +239,14 @@ def main():
     print(f"[bench] workload built in {time.perf_counter() - t0:.1f} s: {desc}", file=sys.stderr, flush=True)
     dA = A.upload()
     dB = dA if B is A else B.upload()
-    sp = ra.SpGEMM(ra.Config(track_stage_times=1))
+    cfg_path = args.config
+    if cfg_path is None and args.workload == "rmat20":
+        cfg_path = os.path.join(ROOT, "research_amd", "tuned", "rmat20.json")
+    knobs = {}
+    if cfg_path and cfg_path != "default":
+        with open(cfg_path) as f:
+            knobs = json.load(f)
+    sp = ra.SpGEMM(ra.Config(track_stage_times=1, **knobs))
 
     for i in range(args.warmup):
         C = sp.run(dA, dB)
@@ -290,7 +303,8 @@ def main():
         "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": desc, "products": products, "nnz_c": st["C"]["nnz"],
-                   "workflow": st["decision"]["workflow"], "parallelism": "single GPU"},
+                   "workflow": st["decision"]["workflow"], "parallelism": "single GPU",
+                   "tuned_config": knobs or None},
         "roofline": roof,
         "whole_product_roofline": {
             "bytes_alg": whole.get("bytes_alg"),
