@@ -34,3 +34,15 @@ def check_product(A, B, C, tol, name="", plain_exact=False):
     nz = np.abs(vals) > 0
     plain = float((err[nz] / np.abs(vals[nz])).max()) if nz.any() else 0.0
     return worst, plain
+
+
+def csr_from_entries(gpu, rows, cols, entries, vtype=0):
+    """Host CSR from (row, col, value) triples, sorted by row, then column."""
+    entries = sorted(entries)
+    ro = np.zeros(rows + 1, np.uint32)
+    for r, _, _ in entries:
+        ro[r + 1] += 1
+    ro = np.cumsum(ro, dtype=np.uint64).astype(np.uint32)
+    ci = np.array([c for _, c, _ in entries], np.uint32)
+    va = np.array([v for _, _, v in entries], np.float32 if vtype else np.float64)
+    return gpu.CSR(rows, cols, ro, ci, va)

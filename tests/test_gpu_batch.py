@@ -164,3 +164,40 @@ def test_batch_long_rows_block_tiles(gpu, mode):
         w, _ = check_product(M, M, Ci, 1e-12, f"{mode} product {i}")
         worst = max(worst, w)
     log(f"{mode}: max normalised error {worst:.2e}")
+
+
+@pytest.mark.parametrize("mode", ["run", "graph"])
+def test_batch_degenerate_products(gpu, mode):
+    """A batch holding a product with no rows, one with rows but no
+    entries, a 1x1 product, one whose A entries meet only empty B rows,
+    and two ordinary products: each extracted product equals the oracle's,
+    for a run and for a captured graph replayed after new values.  A
+    batch of one product is checked too."""
+    from test_gpu_edges import _cases
+    log(f"case: batch of degenerate products ({mode})")
+    cs = _cases(gpu)
+    pairs = [cs[k] for k in ("no_rows", "no_entries", "one_entry", "only_empty_b_rows", "ragged")]
+    # A and B of this pair are separate host objects: new values go to A only
+    pairs.insert(2, (gpu.CSR.rmat(10, 8 << 10, seed=81), gpu.CSR.rmat(10, 8 << 10, seed=81)))
+    As, Bs = [p[0] for p in pairs], [p[1] for p in pairs]
+    dAs, dBs = [A.upload() for A in As], [B.upload() for B in Bs]
+    b = gpu.Batch(gpu.Config())
+    b.set_inputs(dAs, dBs)
+    if mode == "graph":
+        b.plan()
+        b.capture()
+        rng = np.random.default_rng(8)
+        for A, dA in zip(As, dAs):
+            if A.nnz:
+                _set_values(gpu, dA, A, rng.uniform(-1, 1, A.nnz))
+        b.launch()
+    else:
+        b.run()
+    view, rb, cb = b.result()
+    assert rb[-1] == sum(A.rows for A in As) and cb[-1] == sum(B.cols for B in Bs)
+    for i, (A, B) in enumerate(zip(As, Bs)):
+        check_product(A, B, b.extract(i).download(), 1e-12, f"degenerate batch {mode} {i}")
+    one = gpu.Batch(gpu.Config())
+    one.set_inputs([dAs[5]], [dBs[5]])
+    one.run()
+    check_product(As[5], Bs[5], one.extract(0).download(), 1e-12, "batch of one")

@@ -6,22 +6,9 @@ dimension mismatch, which is an error before anything is launched
 import numpy as np
 import pytest
 
-from helpers import check_product, log
+from helpers import check_product, csr_from_entries as _csr, log
 
 pytestmark = pytest.mark.gpu
-
-
-def _csr(gpu, rows, cols, entries, vtype=0):
-    """CSR from (row, col, value) triples (sorted by row, then column)."""
-    entries = sorted(entries)
-    ro = np.zeros(rows + 1, np.uint32)
-    for r, _, _ in entries:
-        ro[r + 1] += 1
-    ro = np.cumsum(ro, dtype=np.uint64).astype(np.uint32)
-    ci = np.array([c for _, c, _ in entries], np.uint32)
-    vdt = np.float32 if vtype else np.float64
-    va = np.array([v for _, _, v in entries], vdt)
-    return gpu.CSR(rows, cols, ro, ci, va)
 
 
 def _cases(gpu):
