@@ -7,7 +7,7 @@ A "step" is one full C = A*A through libocean.so (analysis, HLL sampling,
 binning, symbolic, numeric, output allocation; the output is freed between
 steps).  A and B are resident in HBM before the timed region.  The product
 runs with the configuration the autotuner picked for this matrix
-(research_amd/tuned/rmat20.json, binning thresholds only, the same output;
+(research_amd/tuned/rmat20.json: a symbolic binning threshold, the same output;
 --config default gives the reference's defaults).  Prints one JSON line:
   value     GFLOP/s = 2 * intermediate products / step time
   roofline  dominant kernel family, its algorithmic bytes per launch over its

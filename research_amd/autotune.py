@@ -89,7 +89,11 @@ def _score(w, cfg, reps):
     return float(np.median(times)), h
 
 
-def tune(w, reps: int = 3, passes: int = 2, log=print) -> dict:
+def tune(w, reps: int = 3, passes: int = 2, log=print, gain: float = 0.01) -> dict:
+    """Coordinate descent: a candidate replaces the current configuration
+    when its score is at least `gain` (a fraction) below the current best
+    and its output structure equals the default's.  Repeated scores of one
+    configuration on a warmed handle agree to about 0.3%."""
     cur = {k: v[0] for k, v in SPACE.items()}
     best_ms, ref_hash = _score(w, cur, reps)
     history = [{"config": dict(cur), "ms": best_ms}]
@@ -112,7 +116,7 @@ def tune(w, reps: int = 3, passes: int = 2, log=print) -> dict:
                 if log:
                     log(f"[autotune] pass {p} {knob}={v}: {ms:.2f} ms"
                         f"{'' if ok else ' (structure differs: rejected)'}", flush=True)
-                if ok and ms < best_ms * 0.98:
+                if ok and ms < best_ms * (1.0 - gain):
                     best_ms, cur, changed = ms, cand, True
         if not changed:
             break

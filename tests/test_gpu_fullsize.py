@@ -33,13 +33,20 @@ def _rows_from_device(gpu, dC, off, rows):
 
 
 @pytest.mark.timeout(300)
-def test_rmat20_full_size(gpu):
+@pytest.mark.parametrize("config", ["default", "tuned"])
+def test_rmat20_full_size(gpu, config):
+    """config "tuned": the autotuner's configuration for this matrix, the
+    one the benchmark runs with."""
     gold = json.load(open(os.path.join(HERE, "golden", "rmat20_structure.json")))
-    log("case: rmat 2^20 deg 16 f64 seed 1, C=A*A (full benchmark size)")
+    log(f"case: rmat 2^20 deg 16 f64 seed 1, C=A*A (full benchmark size), {config} config")
     A = gpu.CSR.rmat(20, 16 << 20, 0.57, 0.19, 0.19, vtype=0, seed=1)
     assert A.nnz == gold["a_nnz"]
     dA = A.upload()
-    sp = gpu.SpGEMM(gpu.Config())
+    knobs = {}
+    if config == "tuned":
+        with open(os.path.join(HERE, "..", "research_amd", "tuned", "rmat20.json")) as f:
+            knobs = json.load(f)
+    sp = gpu.SpGEMM(gpu.Config(**knobs))
     dC = sp.run(dA, dA)
     log(f"rmat20: nnz(C)={dC.nnz}, workflow {sp.get_stats()['decision']['workflow']}")
     # structure: sorted, in range, per-row counts and column sets
