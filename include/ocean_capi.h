@@ -153,6 +153,12 @@ void oc_config_default(oc_config *c);
  * `track_stage_time` spelling); unknown keys are ignored. */
 int oc_config_from_json(oc_config *c, const char *json_text);
 int oc_config_load(oc_config *c, const char *path);
+/* Range checks of a configuration (also run by oc_config_from_json and
+ * oc_spgemm_create): expansion coefficients >= 1, fat_ratio >= 1,
+ * HyperLogLog precisions 5 or 6, sample bounds, and the gfx950 kernel
+ * knobs.  The reference applies no checks (src/Utils.cpp:8-37); a value it
+ * would accept and then misbehave on is an OC_ERR_INVALID here. */
+int oc_config_check(const oc_config *c);
 
 /* ---------------------------------------------------------------- SpGEMM */
 /* Replaces: class SpGEMM (kernels/SpGEMM.cuh:209-343):
@@ -410,6 +416,9 @@ int oc_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int oc_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int oc_memset_device(void *p, int value, size_t bytes);
 int oc_device_synchronize(void);
+/* Free and total bytes of the current device (hipMemGetInfo; the
+ * reference's printMemInfo, src/Utils.cpp:59-66, reads cudaMemGetInfo). */
+int oc_device_mem_info(size_t *free_bytes, size_t *total_bytes);
 
 #ifdef __cplusplus
 }

@@ -363,3 +363,9 @@ extern "C" int oc_device_synchronize(void) {
   HIP_TRY(hipDeviceSynchronize());
   return OC_OK;
 }
+
+extern "C" int oc_device_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  OC_REQUIRE(free_bytes && total_bytes, "oc_device_mem_info: null argument");
+  HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+  return OC_OK;
+}

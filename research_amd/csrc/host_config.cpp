@@ -215,6 +215,29 @@ bool as_str(const JValue &v, char *out, size_t cap) {
 
 }  // namespace
 
+extern "C" int oc_config_check(const oc_config *c) {
+  OC_REQUIRE(c, "oc_config_check: null argument");
+  // Bins are sized from products (symbolic) or exact counts (numeric) times
+  // these coefficients; below 1 a row can be given a table smaller than its
+  // output.
+  OC_REQUIRE(c->symbolic_expansion_coe >= 1.0, "symbolic_expansion_coe %g is below 1",
+             c->symbolic_expansion_coe);
+  OC_REQUIRE(c->numeric_expansion_coe >= 1.0, "numeric_expansion_coe %g is below 1",
+             c->numeric_expansion_coe);
+  OC_REQUIRE(c->fat_ratio >= 1, "fat_ratio must be at least 1");
+  OC_REQUIRE((c->hll_precision_low == 5 || c->hll_precision_low == 6) &&
+                 (c->hll_precision_high == 5 || c->hll_precision_high == 6),
+             "HyperLogLog precisions must be 5 or 6");
+  OC_REQUIRE(c->hll_sample_min_rows >= 1 && c->hll_sample_max_rows >= c->hll_sample_min_rows,
+             "need 1 <= hll_sample_min_rows <= hll_sample_max_rows");
+  OC_REQUIRE(c->tile_block == 256 || c->tile_block == 512 || c->tile_block == 1024,
+             "tile_block must be 256, 512 or 1024");
+  OC_REQUIRE(c->tile_log2_cols >= 10 && c->tile_log2_cols <= 16, "tile_log2_cols must be 10..16");
+  OC_REQUIRE(c->band_log2_cols == 0 || (c->band_log2_cols >= 10 && c->band_log2_cols <= 15),
+             "band_log2_cols must be 0 or 10..15");
+  return OC_OK;
+}
+
 extern "C" int oc_config_from_json(oc_config *c, const char *text) {
   OC_REQUIRE(c && text, "oc_config_from_json: null argument");
   Parser ps(text);
@@ -291,7 +314,7 @@ extern "C" int oc_config_from_json(oc_config *c, const char *text) {
     OC_REQUIRE(ok, "config JSON: key \"%s\" has the wrong type", k);
   }
   OC_REQUIRE(*ps.p == '}', "config JSON: unterminated object");
-  return OC_OK;
+  return oc_config_check(c);
 }
 
 extern "C" int oc_config_load(oc_config *c, const char *path) {

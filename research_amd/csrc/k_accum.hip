@@ -771,13 +771,13 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
     const uint32_t r = rows[idx];
     const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1], nA = a1 - a0;
     const uint32_t left = a.left[r], hi = a.right[r];
-    if (nA == 0 || a.products[r] == 0 || hi < left) {
-      if (threadIdx.x == 0) {
-        if (SYM) nnz_sym[r] = 0;
-        else if (o.nnz) o.nnz[r] = 0;
-      }
-      continue;
-    }
+    // No `continue` and one latch after a barrier: with a second back edge
+    // (an early `continue`) the compiler split this loop in two, leaving
+    // thread 0's row claim in an outer loop and the barriers that publish it
+    // in an inner loop the other 1023 threads never leave, so the workgroup
+    // spun on its first row for ever.
+    uint64_t total = 0;
+    if (nA != 0 && a.products[r] != 0 && hi >= left) {
     const uint32_t lo = left & ~31u;
     uint32_t *cur = nA <= kWinLdsCursors ? lcur : scratch + (uint64_t)blockIdx.x * max_nA;
     const uint32_t nwin = (hi - lo) / W + 1;
@@ -786,7 +786,6 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
     int G = 1;
     while (G < 64 && (uint32_t)G < per) G <<= 1;
     const int ngroups = kWinBlock / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
-    uint64_t total = 0;
     const uint64_t off = SYM ? 0 : o.off[r];
     for (uint32_t w = 0; w < nwin; ++w) {
       const uint64_t w0 = (uint64_t)lo + (uint64_t)w * W;
@@ -841,10 +840,12 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
       total += wtotal;
       __syncthreads();
     }
+    }
     if (threadIdx.x == 0) {
       if (SYM) nnz_sym[r] = (uint32_t)total;
       else if (o.nnz) o.nnz[r] = (uint32_t)total;
     }
+    __syncthreads();
   }
 }
 

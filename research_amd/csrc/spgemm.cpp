@@ -979,7 +979,10 @@ int Run::run_precise() {
   }
   fam("sym_dense", "symbolic_dense", false, brows + hoff[7], hoff[14] - hoff[7]);
   mk.mark("symbolic_dense");
-  if (d.use_est_symbolic) {
+  {
+    // Rows whose hash table filled up (the shrunken tables of the estimated
+    // compaction, or a symbolic expansion that undersized a bin) are
+    // recounted with the windowed bitmap, which has no size limit.
     uint32_t n = 0;
     OC_TRY(readback(&n, ovf, 1, s));
     h->sym_overflow_rows = n;
@@ -1297,7 +1300,10 @@ std::string stats_json(const oc_spgemm *h) {
   else workflow = 1;
   j.open("decision");
   j.u64("workflow", workflow);
-  if (workflow <= 0) j.boolean("estimated_symbolic", d.use_est_symbolic);
+  if (workflow <= 0) {
+    j.boolean("estimated_symbolic", d.use_est_symbolic);
+    j.u64("symbolic_overflow_rows", h->sym_overflow_rows);
+  }
   if (workflow <= 1) {
     j.boolean("hash_largest", d.hash_largest);
     j.boolean("dense_bitmap_query", d.dense_query_bitmap);
@@ -1311,6 +1317,8 @@ std::string stats_json(const oc_spgemm *h) {
   j.close();
   if (workflow <= 1) {
     j.open("sampling");
+    j.u64("log2_precision", d.log_p);
+    j.u64("num_samples", d.nsamples);
     j.num("compaction_1", d.comp1);
     j.num("compaction_2", d.comp2);
     if (workflow <= 0) {
@@ -1671,6 +1679,10 @@ extern "C" oc_spgemm *oc_spgemm_create(const oc_config *cfg) {
   }
   if (cfg) h->cfg = *cfg;
   else oc_config_default(&h->cfg);
+  if (oc_config_check(&h->cfg) != OC_OK) {
+    delete h;  // the check's message stays set
+    return nullptr;
+  }
   if (h->cfg.use_learned_binning && h->cfg.learned_model_file[0]) {
     h->owned_model = oc_mlp_create(0);
     if (!h->owned_model || oc_mlp_load(h->owned_model, h->cfg.learned_model_file) != OC_OK) {

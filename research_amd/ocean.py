@@ -107,6 +107,7 @@ def _load():
         "oc_config_default": (None, [P]),
         "oc_config_from_json": (I, [P, C.c_char_p]),
         "oc_config_load": (I, [P, C.c_char_p]),
+        "oc_config_check": (I, [P]),
         "oc_spgemm_create": (P, [P]),
         "oc_spgemm_destroy": (None, [P]),
         "oc_spgemm_run": (I, [P, P, P, P, P]),

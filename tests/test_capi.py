@@ -40,3 +40,25 @@ def test_tables_gfx950():
     assert t["dense_sym"][6] == (163840 - 512) * 8 and t["dense_sym"][0] == 0
     assert t["hash_num"][0] == 0 and t["dense_num"][6] == 0xFFFFFFFF
     assert all(t["hash_num"][b] < t["hash_num"][b + 1] for b in range(1, 6))
+
+
+def test_config_range_checks():
+    """Expansion coefficients below 1 (a bin smaller than its row's output),
+    a zero fat ratio (division), unknown HyperLogLog precisions and kernel
+    knobs outside their ranges are rejected by the check, by the JSON reader
+    and by handle creation."""
+    def cfg(**kw):
+        c = ocean._Config()
+        ocean.lib.oc_config_default(ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+    assert ocean.lib.oc_config_check(ctypes.byref(cfg())) == ocean.OC_OK
+    for kw in (dict(symbolic_expansion_coe=0.9), dict(numeric_expansion_coe=0.5),
+               dict(fat_ratio=0), dict(hll_precision_high=7), dict(hll_sample_min_rows=0),
+               dict(tile_block=300), dict(tile_log2_cols=20), dict(band_log2_cols=7)):
+        assert ocean.lib.oc_config_check(ctypes.byref(cfg(**kw))) == ocean.OC_ERR_INVALID, kw
+    for text in (b'{"tile_block": 300}', b'{"band_log2_cols": 7}'):
+        assert ocean.lib.oc_config_from_json(ctypes.byref(cfg()), text) == ocean.OC_ERR_INVALID
+    assert not ocean.lib.oc_spgemm_create(ctypes.byref(cfg(symbolic_expansion_coe=0.75)))
+    assert b"symbolic_expansion_coe" in ocean.lib.oc_last_error()

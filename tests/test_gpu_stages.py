@@ -100,10 +100,15 @@ def test_murmur3_cols(gpu):
 
 
 @pytest.mark.parametrize("log_p", [5, 6])
-@pytest.mark.parametrize("name", ["rmat13", "er", "rect"])
+@pytest.mark.parametrize("name", ["rmat13", "er", "rect", "hubs"])
 def test_hll_build_estimate_sample(gpu, name, log_p):
+    """Sketches, per-row estimates and samples bit-exact.  `hubs` has rows
+    of 2049, 6000 and 40000 entries: rows above 2048 entries are merged by
+    a whole workgroup each (the heavy-row kernel), in both modes."""
     log(f"case: hll {name} log_p={log_p}")
     A, B = _pair(gpu, name)
+    if name == "hubs":
+        assert np.diff(A.row_offsets.astype(np.int64)).max() > 2048
     dA, dB = A.upload(), B.upload()
     lib = gpu.lib
     P = 1 << log_p
@@ -118,7 +123,15 @@ def test_hll_build_estimate_sample(gpu, name, log_p):
         _chk(gpu, lib.oc_hll_estimate(log_p, C.byref(dA.m), hb.ptr, expansion, est.ptr, None))
         want = ora.hll_merge(log_p, A, ref_b, expansion, sizes)
         assert np.array_equal(est.to_array(np.uint32, A.rows), want), expansion
-    for ns in (1, 648, 3000):
+    samples = (1, 648, 3000)
+    if name == "hubs":
+        # enough samples that the LCG reaches rows above 2048 entries
+        heavy = np.nonzero(np.diff(A.row_offsets.astype(np.int64)) > 2048)[0]
+        ns = 1
+        while not np.isin([ora.sample_row(i, A.rows) for i in range(ns)], heavy).any():
+            ns *= 2
+        samples = samples + (ns,)
+    for ns in samples:
         rows, se = _out(gpu, 4 * ns), _out(gpu, 4 * ns)
         _chk(gpu, lib.oc_hll_sample(log_p, C.byref(dA.m), hb.ptr, ns, rows.ptr, se.ptr, None))
         wr, we = ora.hll_sample(log_p, A, ref_b, ns)
