@@ -130,7 +130,9 @@ typedef struct oc_config {
   int32_t use_learned_binning;
   char learned_model_file[512];
   /* gfx950 kernel choice for rows beyond the LDS hash/dense kernels:
-   * 2 rank-compressed tiles (default), 1 dense tiles, 0 cursor windows. */
+   * 3 one-gather rank tiles (the symbolic pass stores the long rows'
+   * bitmaps), 2 rank-compressed tiles (default), 1 dense tiles, 0 cursor
+   * windows. */
   int32_t use_tiled_numeric;
   int32_t tile_log2_cols;  /* tile width 2^k columns (10..16), default 14 */
   int32_t tile_block;      /* workgroup size of the tiled kernel: 256/512/1024 */

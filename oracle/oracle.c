@@ -630,3 +630,81 @@ void ora_numeric_rows(uint64_t nrows, const uint32_t *rows, int vtype,
     free(touched);
   }
 }
+
+/* CPU baseline, not a checker: the plain row-wise Gustavson a CPU
+ * library runs.  Dense FP64 accumulator with a touched list (no
+ * compensation, no |a*b| sums); a row's columns come out in order by a
+ * scan of the touched range when it is dense (at least 1/16 of it
+ * touched), else by sorting the touched list; each thread appends its
+ * rows to its own growing output buffer.  Returns nnz of rows [r0, r1);
+ * *checksum receives the sum of all values (keeps the work observable). */
+uint64_t ora_baseline_spgemm(uint64_t r0, uint64_t r1, int vtype,
+                             const uint32_t *a_rowptr, const uint32_t *a_cols,
+                             const void *a_vals, uint64_t b_cols_n,
+                             const uint32_t *b_rowptr, const uint32_t *b_cols,
+                             const void *b_vals, double *checksum, int nthreads) {
+  int nt = nthreads_or_default(nthreads);
+  size_t nc = (size_t)(b_cols_n ? b_cols_n : 1);
+  uint64_t total = 0;
+  double sum = 0.0;
+#pragma omp parallel num_threads(nt) reduction(+ : total, sum)
+  {
+    double *acc = (double *)malloc(sizeof(double) * nc);
+    uint8_t *flag = (uint8_t *)calloc(nc, 1);
+    uint32_t *touched = (uint32_t *)malloc(sizeof(uint32_t) * nc);
+    size_t cap = 1 << 20, used = 0;
+    uint32_t *oc = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+    double *ov = (double *)malloc(sizeof(double) * cap);
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t r = (int64_t)r0; r < (int64_t)r1; ++r) {
+      uint32_t n = 0, lo = UINT32_MAX, hi = 0;
+      for (uint32_t j = a_rowptr[r]; j < a_rowptr[r + 1]; ++j) {
+        uint32_t k = a_cols[j];
+        double av = ld_val(vtype, a_vals, j);
+        for (uint32_t q = b_rowptr[k]; q < b_rowptr[k + 1]; ++q) {
+          uint32_t c = b_cols[q];
+          double p = av * ld_val(vtype, b_vals, q);
+          if (!flag[c]) {
+            flag[c] = 1;
+            touched[n++] = c;
+            acc[c] = p;
+            if (c < lo) lo = c;
+            if (c > hi) hi = c;
+          } else {
+            acc[c] += p;
+          }
+        }
+      }
+      if (used + n > cap) {
+        while (used + n > cap) cap *= 2;
+        oc = (uint32_t *)realloc(oc, sizeof(uint32_t) * cap);
+        ov = (double *)realloc(ov, sizeof(double) * cap);
+      }
+      if (n && (uint64_t)n * 16 >= (uint64_t)(hi - lo + 1)) {
+        uint32_t t = 0;
+        for (uint32_t c = lo; c <= hi; ++c)
+          if (flag[c]) touched[t++] = c;
+      } else {
+        qsort(touched, n, sizeof(uint32_t), cmp_u32);
+      }
+      for (uint32_t t = 0; t < n; ++t) {
+        uint32_t c = touched[t];
+        oc[used] = c;
+        ov[used] = acc[c];
+        sum += acc[c];
+        ++used;
+        flag[c] = 0;
+      }
+      total += n;
+      /* the buffer is reused: a caller would hand each row range on */
+      if (used > (1u << 24)) used = 0;
+    }
+    free(acc);
+    free(flag);
+    free(touched);
+    free(oc);
+    free(ov);
+  }
+  if (checksum) *checksum = sum;
+  return total;
+}

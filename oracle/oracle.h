@@ -171,6 +171,15 @@ uint64_t ora_symbolic_rows(uint64_t nrows, const uint32_t *rows,
                            const uint32_t *b_cols, uint32_t *nnz_row,
                            int nthreads);
 
+/* CPU baseline (not a checker): plain row-wise Gustavson, dense FP64
+ * accumulator + touched list, rows in column order, per-thread output
+ * buffers.  Returns nnz of rows [r0, r1); *checksum = sum of values. */
+uint64_t ora_baseline_spgemm(uint64_t r0, uint64_t r1, int vtype,
+                             const uint32_t *a_rowptr, const uint32_t *a_cols,
+                             const void *a_vals, uint64_t b_cols_n,
+                             const uint32_t *b_rowptr, const uint32_t *b_cols,
+                             const void *b_vals, double *checksum, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,7 @@ def _load():
         "ora_numeric": (None, [U64, U64, I, P, P, P, U64, P, P, P, P, P, P, P, I, I]),
         "ora_numeric_rows": (None, [U64, P, I, P, P, P, U64, P, P, P, P, P, P, P, I]),
         "ora_symbolic_rows": (U64, [U64, P, P, P, U64, P, P, P, I]),
+        "ora_baseline_spgemm": (U64, [U64, U64, I, P, P, P, U64, P, P, P, P, I]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -234,6 +235,19 @@ def numeric(A, B, r0=0, r1=None, compensated=1, with_cond=True, nthreads=0):
                     _p(rowptr), _p(cols), _p(vals), _p(cond) if cond is not None else None,
                     compensated, nthreads)
     return rowptr, cols[:n], vals[:n], (cond[:n] if cond is not None else None)
+
+
+def baseline_spgemm(A, B, r0=0, r1=None, nthreads=0):
+    """CPU baseline (not a checker): plain row-wise Gustavson over rows
+    [r0, r1) of A*B.  Returns (nnz, sum of the values)."""
+    r1 = A.rows if r1 is None else r1
+    arp, aci, brp, bci = _u32(A.row_offsets), _u32(A.col_ids), _u32(B.row_offsets), _u32(B.col_ids)
+    av, vt = _vals(A.values)
+    bv, _ = _vals(B.values)
+    cs = C.c_double(0.0)
+    n = lib.ora_baseline_spgemm(r0, r1, vt, _p(arp), _p(aci), _p(av), B.cols, _p(brp), _p(bci),
+                                _p(bv), C.byref(cs), nthreads)
+    return int(n), cs.value
 
 
 def numeric_rows(A, B, rows, nthreads=0):

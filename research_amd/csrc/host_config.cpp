@@ -230,6 +230,8 @@ extern "C" int oc_config_check(const oc_config *c) {
              "HyperLogLog precisions must be 5 or 6");
   OC_REQUIRE(c->hll_sample_min_rows >= 1 && c->hll_sample_max_rows >= c->hll_sample_min_rows,
              "need 1 <= hll_sample_min_rows <= hll_sample_max_rows");
+  OC_REQUIRE(c->use_tiled_numeric >= 0 && c->use_tiled_numeric <= 3,
+             "use_tiled_numeric must be 0..3");
   OC_REQUIRE(c->tile_block == 256 || c->tile_block == 512 || c->tile_block == 1024,
              "tile_block must be 256, 512 or 1024");
   OC_REQUIRE(c->tile_log2_cols >= 10 && c->tile_log2_cols <= 16, "tile_log2_cols must be 10..16");

@@ -16,83 +16,9 @@
 #include <cstring>
 #include <vector>
 
-#include "dev_common.hpp"
-#include "launch.hpp"
+#include "k_common.hpp"
 
 namespace oc {
-
-#define HIP_TRY(call)                                                         \
-  do {                                                                        \
-    hipError_t e_ = (call);                                                   \
-    if (e_ != hipSuccess) {                                                   \
-      set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                    \
-                hipGetErrorString(e_));                                       \
-      return OC_ERR_HIP;                                                      \
-    }                                                                         \
-  } while (0)
-#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
-
-static inline uint32_t cdiv(uint64_t a, uint64_t b) {
-  return (uint32_t)((a + b - 1) / b);
-}
-
-template <typename T>
-struct KArgs {
-  const uint32_t *__restrict__ a_rp;
-  const uint32_t *__restrict__ a_ci;
-  const T *__restrict__ a_v;
-  const uint32_t *__restrict__ b_rp;
-  const uint32_t *__restrict__ b_ci;
-  const T *__restrict__ b_v;
-  uint32_t b_cols;
-  const uint32_t *__restrict__ products;
-  const uint32_t *__restrict__ max_b;
-  const uint32_t *__restrict__ left;
-  const uint32_t *__restrict__ right;
-};
-
-template <typename T>
-struct KOut {
-  const uint64_t *off;
-  uint32_t *cols;
-  T *vals;
-  uint32_t *nnz;
-  const uint32_t *cap;
-  uint32_t *ovf_rows;
-  uint32_t *ovf_count;
-  const uint32_t *nnz_hint;
-};
-
-template <typename T>
-static KArgs<T> kargs(const AccArgs &a) {
-  KArgs<T> k;
-  k.a_rp = a.A.rp;
-  k.a_ci = a.A.ci;
-  k.a_v = (const T *)a.A.v;
-  k.b_rp = a.B.rp;
-  k.b_ci = a.B.ci;
-  k.b_v = (const T *)a.B.v;
-  k.b_cols = (uint32_t)a.B.cols;
-  k.products = a.products;
-  k.max_b = a.max_b;
-  k.left = a.left;
-  k.right = a.right;
-  return k;
-}
-
-template <typename T>
-static KOut<T> kout(const NumOut &o) {
-  KOut<T> k;
-  k.off = o.off;
-  k.cols = o.cols;
-  k.vals = (T *)o.vals;
-  k.nnz = o.nnz;
-  k.cap = o.cap;
-  k.ovf_rows = o.ovf_rows;
-  k.ovf_count = o.ovf_count;
-  k.nnz_hint = o.nnz_hint;
-  return k;
-}
 
 static int key_bits_for(uint64_t cols) {
   int b = 0;
@@ -121,10 +47,6 @@ __device__ __forceinline__ int pick_group(uint32_t nA, uint32_t prod,
 
 __device__ __forceinline__ uint32_t lds_load(uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-__device__ __forceinline__ void lds_add_f64(double *p, double v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Walk one B row segment [s, e) with a group of G lanes (q = s + lig, step
@@ -452,20 +374,26 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
 // ========================================================= symbolic dense
 // Exact row nnz from an LDS bitmap over [left, right]
 // (kernels/AccumulatorDense.cuh:43-126).
+// bmoff/bm (optional): rows with bmoff[r] != ~0 also store their bitmap,
+// n4 uint4 groups from column left & ~127, at bm + bmoff[r], for the
+// one-gather rank tiles of the numeric pass.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
                                                   const uint32_t *__restrict__ rows,
-                                                  uint32_t *__restrict__ nnz) {
+                                                  uint32_t *__restrict__ nnz,
+                                                  const uint64_t *__restrict__ bmoff,
+                                                  uint4 *__restrict__ bm) {
   extern __shared__ uint32_t lds_u32[];
   uint32_t *hdr = lds_u32;
   uint32_t *bits = lds_u32 + 64;
   const uint32_t r = rows[blockIdx.x];
   const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
-  const uint32_t lo = a.left[r] & ~31u, hi = a.right[r];
+  const uint32_t lo = a.left[r] & ~127u, hi = a.right[r];
   if (a1 == a0 || a.products[r] == 0 || hi < a.left[r]) {
     if (threadIdx.x == 0) nnz[r] = 0;
     return;
   }
+  const uint64_t bo = bmoff ? bmoff[r] : ~0ull;
   // the bitmap is cleared and counted 128 bits at a time (the launch
   // allocates whole 16-byte groups)
   const uint32_t nwords = ((hi - lo) >> 5) + 1, n4 = (nwords + 3) / 4;
@@ -497,6 +425,7 @@ __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
   for (uint32_t i = threadIdx.x; i < n4; i += BS) {
     const uint4 v = bits4[i];
     cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+    if (bo != ~0ull) bm[bo + i] = v;
   }
   cnt = group_sum(cnt, kWave);
   if (lane_id() == 0) atomicAdd(&hdr[0], cnt);
@@ -849,30 +778,10 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
   }
 }
 
-template <typename K>
-static int set_lds(K kernel, size_t bytes) {
-  if (bytes > 65536)
-    HIP_TRY(hipFuncSetAttribute((const void *)kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)bytes));
-  return OC_OK;
-}
-
 // ================================================================= tiled
 // Column tiles of W = 2^logW columns.  ts[k * T1 + t] is the index of the
 // first entry of B row k whose column is >= t * W (T1 = tiles + 1), so the
 // part of any B row inside any tile is known without walking the row.
-// Clears work counters with a kernel rather than a memset node, so the
-// launch sequence can be captured into a HIP graph and replayed.
-__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
-}
-static int zero_u32(uint32_t *p, uint32_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p, n);
-  LAUNCH_CHECK();
-  return OC_OK;
-}
-
 __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t *__restrict__ b_rp,
                                                      const uint32_t *__restrict__ b_ci,
                                                      uint32_t rows, uint32_t logW,
@@ -1079,8 +988,6 @@ struct RankShape {
 // position its A entry with one LDS read, and lane-strided positions keep
 // the B gathers coalesced.  The next round's map is built while a round's
 // gathers are in flight.
-// Entries per B row of the compact tile-start table of the widest tile.
-constexpr uint32_t kWideStarts = 16;
 // Owner-map entries: A entry id in the low bits, map tag above it.
 constexpr uint32_t kRankIdBits = 10, kRankIdMask = (1u << kRankIdBits) - 1;
 constexpr uint32_t kRankTags = 1u << (16 - kRankIdBits);
@@ -1659,17 +1566,18 @@ int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
 }
 
 int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
-                     uint32_t nrows, uint32_t *nnz, hipStream_t s) {
+                     uint32_t nrows, uint32_t *nnz, hipStream_t s, const uint64_t *bmoff,
+                     void *bm) {
   if (nrows == 0) return OC_OK;
   const Tables &t = tables();
-  // bitmap words of the widest span, rounded up to whole 16-byte groups
-  const size_t lds = 256 + (size_t)((t.dense_sym[dense_bin] / 32 + 2 + 3) & ~3u) * 4;
+  // whole 16-byte groups from column left & ~127 over the widest span
+  const size_t lds = 256 + ((size_t)(t.dense_sym[dense_bin] + 126) / 128 + 1) * 16;
   KArgs<float> k = kargs<float>(a);
 #define GO(BS)                                                                 \
   do {                                                                         \
     OC_TRY(set_lds(k_sym_dense<BS>, lds));                                     \
     hipLaunchKernelGGL(k_sym_dense<BS>, dim3(nrows), dim3(BS), lds, s, k, rows, \
-                       nnz);                                                   \
+                       nnz, bmoff, (uint4 *)bm);                               \
   } while (0)
   BIN_DISPATCH(dense_bin, GO)
 #undef GO

@@ -240,6 +240,14 @@ __global__ __launch_bounds__(kLongBlock) void k_analysis_long(
   }
 }
 
+__global__ void k_zero3(uint32_t *a, uint32_t *b, uint32_t *c) {
+  if (threadIdx.x == 0) {
+    if (a) *a = 0;
+    if (b) *b = 0;
+    if (c) *c = 0;
+  }
+}
+
 size_t analysis_temp_bytes(uint64_t a_rows, uint64_t a_nnz, uint64_t b_rows) {
   const uint64_t nlong = std::min<uint64_t>(a_rows, a_nnz / (kLongAIters * 4) + 1);
   return (b_rows + 1) * 16 + 256 + nlong * 4;
@@ -248,7 +256,13 @@ size_t analysis_temp_bytes(uint64_t a_rows, uint64_t a_nnz, uint64_t b_rows) {
 int launch_analysis(const DevCsr &A, const DevCsr &B, const RowInfo &ri,
                     uint32_t *max_a_nnz, uint32_t *max_products, void *temp,
                     hipStream_t s) {
-  if (A.rows == 0) return OC_OK;
+  if (A.rows == 0) {
+    if (max_a_nnz || max_products) {
+      hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, s, nullptr, max_a_nnz, max_products);
+      LAUNCH_CHECK();
+    }
+    return OC_OK;
+  }
   OC_REQUIRE(A.rows < (1ull << 32) && B.rows < (1ull << 32),
              "analysis: more than 2^32 - 1 rows");
   void *own = nullptr;
@@ -259,9 +273,9 @@ int launch_analysis(const DevCsr &A, const DevCsr &B, const RowInfo &ri,
   uint4 *info = (uint4 *)temp;
   uint32_t *long_count = (uint32_t *)(info + B.rows + 1);
   uint32_t *long_rows = long_count + 64;
-  HIP_TRY(hipMemsetAsync(long_count, 0, 4, s));
-  if (max_a_nnz) HIP_TRY(hipMemsetAsync(max_a_nnz, 0, 4, s));
-  if (max_products) HIP_TRY(hipMemsetAsync(max_products, 0, 4, s));
+  // counters cleared by a kernel, not memset nodes (capture-safe)
+  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, s, long_count, max_a_nnz, max_products);
+  LAUNCH_CHECK();
   if (B.rows)
     hipLaunchKernelGGL(k_brow_info, dim3(std::min<uint64_t>(cdiv(B.rows, 256), 1u << 20)),
                        dim3(256), 0, s, B.rp, B.ci, (uint32_t)B.rows, info);
@@ -352,8 +366,10 @@ size_t scan_temp_bytes(uint64_t n) { return (size_t)(cdiv(n, kScanTile) + 1) * 8
 
 int launch_scan_u32(const uint32_t *in, uint64_t *out, uint64_t n, void *temp,
                     hipStream_t s) {
-  if (n == 0) {
-    HIP_TRY(hipMemsetAsync(out, 0, 8, s));
+  if (n == 0) {  // out[0] = 0 by a kernel, not a memset node (capture-safe)
+    hipLaunchKernelGGL(k_scan_final, dim3(1), dim3(kScanBlock), 0, s, in, out,
+                       (uint64_t)0, (const uint64_t *)nullptr, 1);
+    LAUNCH_CHECK();
     return OC_OK;
   }
   uint32_t nb = cdiv(n, kScanTile);

@@ -137,8 +137,11 @@ struct NumOut {
 int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
                     uint32_t nrows, uint32_t limit, uint32_t *nnz,
                     uint32_t *ovf_rows, uint32_t *ovf_count, hipStream_t s);
+// bmoff/bm (optional): rows with bmoff[r] != ~0 store their bitmap (uint4
+// groups from column left & ~127) at bm + bmoff[r] (launch_bitmap_offsets)
 int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
-                     uint32_t nrows, uint32_t *nnz, hipStream_t s);
+                     uint32_t nrows, uint32_t *nnz, hipStream_t s,
+                     const uint64_t *bmoff = nullptr, void *bm = nullptr);
 // Windowed kernels take the bin row list (device) and its host-known
 // length; cursor scratch is grid * max_a_nnz u32.
 int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
@@ -174,6 +177,22 @@ int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row);
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
                     const uint32_t *ts, uint32_t T1, const uint32_t *tsw, const uint32_t *bbase,
                     uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s);
+// One-gather rank tiles (k_rank.hip), 1024 threads, tiles up to
+// 2^rank1_wide_log() columns; tsw only when built at that width.  Rows
+// with a stored bitmap (bmoff[r] != ~0, no block bases) gather each
+// product once; the others mark their tiles first.
+uint32_t rank1_wide_log();
+int launch_num_rank1(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
+                     const uint32_t *ts, uint32_t T1, const uint32_t *tsw,
+                     const uint32_t *bbase, const void *bm, const uint64_t *bmoff,
+                     const NumOut &o, void *scratch, hipStream_t s);
+// Stored-bitmap offsets (uint4 units) for the listed rows with at least
+// min_products products; bmoff[0..m) = ~0 elsewhere; offs[n] = total.
+// sz: n u32, offs: n + 1 u64, scan_tmp: scan_temp_bytes(n).
+int launch_bitmap_offsets(const uint32_t *rows, uint32_t n, uint64_t m,
+                          const uint32_t *products, const uint32_t *left,
+                          const uint32_t *right, uint32_t min_products, uint32_t *sz,
+                          uint64_t *offs, void *scan_tmp, uint64_t *bmoff, hipStream_t s);
 // Ultra-sparse expand-sort-compress for all rows with products <= esc_n.
 int launch_esc(int esc_n, const AccArgs &a, const NumOut &o, hipStream_t s);
 

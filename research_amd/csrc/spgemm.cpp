@@ -200,6 +200,8 @@ struct oc_spgemm {
   uint64_t total_est = 0;
   uint64_t est_spare_need = 0;       // entries the overflow rows reserved
   bool est_separate_spare = false;   // ... in a buffer of their own
+  bool est_memory_fallback = false;
+  uint64_t bitmap_bytes = 0;         // row bitmaps stored by the symbolic pass  // estimated region + C beyond the device: precise instead
   uint32_t overflow_rows = 0, sym_overflow_rows = 0, precise_recount = 0;
   int vtype = OC_F64;
   double total_ms = 0;
@@ -232,6 +234,8 @@ struct oc_spgemm {
     NumOut o{};
     uint32_t *tile_ts = nullptr, *tile_tsw = nullptr;
     const uint32_t *block_base = nullptr;
+    void *row_bm = nullptr;
+    const uint64_t *row_bmoff = nullptr;
     uint32_t tile_logw = 0, tile_t1 = 0, max_a_nnz = 0;
     int tiles_ok = -1;
     void *win_scratch = nullptr;
@@ -366,6 +370,9 @@ struct Run {
   const uint32_t *block_base = nullptr;  // per-B-row block bases of the tables
   uint32_t tile_logw = 0, tile_t1 = 0;
   int tiles_ok = -1;
+  // row bitmaps stored by the dense symbolic pass (one-gather rank tiles)
+  void *row_bm = nullptr;
+  const uint64_t *row_bmoff = nullptr;
   // long rows in column bands (precise workflow)
   struct Band {
     uint32_t n = 0, nbands = 0, logw = 0;
@@ -420,15 +427,17 @@ struct Run {
     const bool blocks = h->b_block_base != nullptr && h->b_block_rp == (const void *)B.rp;
     block_base = blocks ? h->b_block_base : nullptr;
     const uint64_t width = blocks ? h->b_block_span : B.cols;
-    tiles_ok = (h->cfg.use_tiled_numeric == 2 || (h->cfg.use_tiled_numeric == 1 && !blocks)) &&
+    const int mode = h->cfg.use_tiled_numeric;
+    tiles_ok = (mode == 2 || mode == 3 || (mode == 1 && !blocks)) &&
                tile_plan(B.rows, width, (uint32_t)h->cfg.tile_log2_cols,
                          (uint32_t)h->cfg.tile_block, &tile_logw, &tile_t1, &bytes);
     if (tiles_ok) {
       OC_TRY(ar.get(&tile_ts, bytes / 4));
       OC_TRY(launch_tile_starts(B, tile_logw, tile_t1, tile_ts, s, block_base));
       size_t per = 0;
-      const uint32_t bs = (uint32_t)h->cfg.tile_block;
-      if (h->cfg.use_tiled_numeric == 2 && wide_starts_plan(width, bs, &per)) {
+      // the one-gather kernel always runs 1024-thread workgroups
+      const uint32_t bs = mode == 3 ? 1024u : (uint32_t)h->cfg.tile_block;
+      if (mode >= 2 && wide_starts_plan(width, bs, &per)) {
         OC_TRY(ar.get(&tile_tsw, B.rows * per));
         OC_TRY(launch_tile_starts(B, rank_wide_log(bs), (uint32_t)per, tile_tsw, s,
                                   block_base));
@@ -438,10 +447,44 @@ struct Run {
     return OC_OK;
   }
 
+  // Row bitmaps of the dense symbolic bins (rows with at least
+  // kBitmapMinProducts products), stored by the symbolic pass for the
+  // one-gather rank tiles of the numeric pass.  Skipped for block-diagonal
+  // batches and when the bitmaps would take more than a fifth of the device.
+  int row_bitmaps(const uint32_t *brows, const uint32_t *hoff) {
+    constexpr uint32_t kBitmapMinProducts = 2048;
+    h->bitmap_bytes = 0;
+    const bool blocks = h->b_block_base != nullptr && h->b_block_rp == (const void *)B.rp;
+    if (h->cfg.use_tiled_numeric != 3 || blocks || hoff[14] <= hoff[8]) return OC_OK;
+    const uint32_t nd = hoff[14] - hoff[8];
+    uint64_t *bmoff, *offs;
+    uint32_t *sz;
+    OC_TRY(ar.get(&bmoff, A.rows));
+    OC_TRY(ar.get(&sz, nd));
+    OC_TRY(ar.get(&offs, (uint64_t)nd + 1));
+    OC_TRY(launch_bitmap_offsets(brows + hoff[8], nd, A.rows, ri.products, ri.left, ri.right,
+                                 kBitmapMinProducts, sz, offs, scan_tmp, bmoff, s));
+    uint64_t n4 = 0;
+    OC_TRY(readback(&n4, offs + nd, 1, s));
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    if (n4 == 0 || (double)n4 * 16 > 0.2 * (double)tot) return OC_OK;
+    uint8_t *bm;
+    OC_TRY(ar.get(&bm, n4 * 16));
+    row_bm = bm;
+    row_bmoff = bmoff;
+    h->bitmap_bytes = n4 * 16;
+    return OC_OK;
+  }
+
   int big_rows(const uint32_t *rows, uint32_t n, const NumOut &o) {
     if (!n) return OC_OK;
     OC_TRY(window_scratch());
     OC_TRY(tiles());
+    if (tiles_ok && h->cfg.use_tiled_numeric == 3)
+      return launch_num_rank1(acc(), rows, n, tile_logw, tile_ts, tile_t1,
+                              rank_wide_log(1024) == rank1_wide_log() ? tile_tsw : nullptr,
+                              block_base, row_bm, row_bmoff, o, win_scratch, s);
     if (tiles_ok && h->cfg.use_tiled_numeric == 2)
       return launch_num_rank(acc(), rows, n, tile_logw, tile_ts, tile_t1, tile_tsw,
                              block_base, (uint32_t)h->cfg.tile_products_cap,
@@ -952,6 +995,7 @@ int Run::run_precise() {
   mk.mark("symbolic_binning");
   const AccArgs a = acc();
   const Tables &t = tables();
+  OC_TRY(row_bitmaps(brows, hoff));
   for (int b = 0; b <= 5; ++b) {
     const uint32_t n = hoff[b + 1] - hoff[b];
     if (!n) continue;
@@ -975,7 +1019,7 @@ int Run::run_precise() {
       set_error("internal: dense symbolic bin 0 is not used");
       return OC_ERR_INVALID;
     }
-    OC_TRY(launch_sym_dense(b - 7, a, brows + hoff[b], n, nnz, s));
+    OC_TRY(launch_sym_dense(b - 7, a, brows + hoff[b], n, nnz, s, row_bmoff, row_bm));
   }
   fam("sym_dense", "symbolic_dense", false, brows + hoff[7], hoff[14] - hoff[7]);
   mk.mark("symbolic_dense");
@@ -1038,6 +1082,8 @@ int Run::run_precise() {
     p.tile_ts = tile_ts;
     p.tile_tsw = tile_tsw;
     p.block_base = block_base;
+    p.row_bm = row_bm;
+    p.row_bmoff = row_bmoff;
     p.tile_logw = tile_logw;
     p.tile_t1 = tile_t1;
     p.tiles_ok = tiles_ok;
@@ -1108,6 +1154,20 @@ int Run::run_est() {
   // (SpGEMM.cuh:1560-1587)
   const double coe = c.hll_numeric_malloc_expansion_coe;
   const uint64_t cap_total = std::max<uint64_t>((uint64_t)((double)h->total_est * coe), h->total_est);
+  {
+    // The estimated region and C are held at once (C is at least the
+    // estimate's size).  When both would take more than 60% of the device,
+    // the precise workflow, which holds only C, runs instead.
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    const double bytes = (double)(cap_total + h->total_est) * (double)(4 + vsize(A.vtype));
+    if (bytes > 0.6 * (double)tot) {
+      h->est_memory_fallback = true;
+      h->d.ana2_type = 0;
+      h->learned_run = false;
+      return run_precise();
+    }
+  }
   uint32_t *tc;
   uint8_t *tv;
   OC_TRY(ar.get(&tc, cap_total));
@@ -1129,7 +1189,7 @@ int Run::run_est() {
   // rows go to the spare region with the overflow rows.
   const uint32_t n13 = hoff[14] - hoff[13];
   if (n13) OC_TRY(tiles());
-  const bool inplace13 = n13 && tiles_ok && c.use_tiled_numeric == 2;
+  const bool inplace13 = n13 && tiles_ok && c.use_tiled_numeric >= 2;
   if (inplace13) {
     OC_TRY(big_rows(brows + hoff[13], n13, o));
     fam("num_window", "numeric_window", true, brows + hoff[13], n13);
@@ -1303,12 +1363,14 @@ std::string stats_json(const oc_spgemm *h) {
   if (workflow <= 0) {
     j.boolean("estimated_symbolic", d.use_est_symbolic);
     j.u64("symbolic_overflow_rows", h->sym_overflow_rows);
+    j.u64("stored_row_bitmap_bytes", h->bitmap_bytes);
   }
   if (workflow <= 1) {
     j.boolean("hash_largest", d.hash_largest);
     j.boolean("dense_bitmap_query", d.dense_query_bitmap);
   }
   j.boolean("learned_estimates", h->learned_run);
+  j.boolean("estimation_memory_fallback", h->est_memory_fallback);
   if (h->learned_consulted) {
     j.u64("learned_total_capacity", h->learned_total);
     j.num("learned_products_per_capacity",
@@ -1512,6 +1574,8 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
   h->learned_run = h->learned_consulted = false;
   h->est_spare_need = 0;
   h->est_separate_spare = false;
+  h->est_memory_fallback = false;
+  h->bitmap_bytes = 0;
   h->learned_total = 0;
   h->stage_ms.clear();
   std::memset(dC, 0, sizeof(*dC));
@@ -1628,6 +1692,8 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   r.tile_ts = p.tile_ts;
   r.tile_tsw = p.tile_tsw;
   r.block_base = p.block_base;
+  r.row_bm = p.row_bm;
+  r.row_bmoff = p.row_bmoff;
   r.tile_logw = p.tile_logw;
   r.tile_t1 = p.tile_t1;
   r.tiles_ok = p.tiles_ok;

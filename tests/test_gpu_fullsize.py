@@ -33,19 +33,22 @@ def _rows_from_device(gpu, dC, off, rows):
 
 
 @pytest.mark.timeout(300)
-@pytest.mark.parametrize("config", ["default", "tuned"])
+@pytest.mark.parametrize("config", ["default", "tuned", "one_gather"])
 def test_rmat20_full_size(gpu, config):
     """config "tuned": the autotuner's configuration for this matrix, the
-    one the benchmark runs with."""
+    one the benchmark runs with; "one_gather": the same with the long rows
+    through the one-gather rank tiles (use_tiled_numeric 3)."""
     gold = json.load(open(os.path.join(HERE, "golden", "rmat20_structure.json")))
     log(f"case: rmat 2^20 deg 16 f64 seed 1, C=A*A (full benchmark size), {config} config")
     A = gpu.CSR.rmat(20, 16 << 20, 0.57, 0.19, 0.19, vtype=0, seed=1)
     assert A.nnz == gold["a_nnz"]
     dA = A.upload()
     knobs = {}
-    if config == "tuned":
+    if config in ("tuned", "one_gather"):
         with open(os.path.join(HERE, "..", "research_amd", "tuned", "rmat20.json")) as f:
             knobs = json.load(f)
+    if config == "one_gather":
+        knobs["use_tiled_numeric"] = 3
     sp = gpu.SpGEMM(gpu.Config(**knobs))
     dC = sp.run(dA, dA)
     log(f"rmat20: nnz(C)={dC.nnz}, workflow {sp.get_stats()['decision']['workflow']}")

@@ -160,3 +160,45 @@ def test_training_and_learned_spgemm(gpu):
     assert st2["decision"]["learned_estimates"] == (ratio >= 1.5)
     log(f"learned rule: products / capacity {ratio:.3f} -> workflow {st2['decision']['workflow']}")
     check_product(A, A, dC2.download(), 1e-12, "learned rmat13, default rule")
+
+
+def test_learned_gain_on_batch(gpu):
+    """What the trained network gains: one warmed Batch handle over 256
+    mixed products (the batched workload's generator), timed with the
+    HyperLogLog-sampled path and then with the trained model's capacities;
+    both give the same structure, and the learned run is faster."""
+    import os
+    import time
+    from research_amd import predictor
+    from research_amd.workloads import batched_products
+    import research_amd as ra
+    log("case: learned gain on a batch of 256 mixed products")
+    mats = batched_products(ra, 256)
+    dev = [M.upload() for M in mats]
+    b = gpu.Batch(gpu.Config())
+    b.set_inputs(dev)
+
+    def timed(reps=5):
+        b.run()
+        gpu._check(gpu.lib.oc_device_synchronize())
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            b.run()
+            gpu._check(gpu.lib.oc_device_synchronize())
+            ts.append(time.perf_counter() - t0)
+        view, _, _ = b.result()
+        return float(np.median(ts)), view.row_set_hash(), b.get_stats()
+
+    t_hll, h_hll, st_hll = timed()
+    m = gpu.Mlp(0)
+    assert os.path.exists(predictor.DEFAULT_MODEL)
+    m.load(predictor.DEFAULT_MODEL)
+    gpu._check(gpu.lib.oc_spgemm_set_model(gpu.lib.oc_batch_handle(b.h), m.h, 0.99))
+    t_lrn, h_lrn, st_lrn = timed()
+    assert np.array_equal(h_hll, h_lrn), "learned capacities changed the structure"
+    gain = t_hll / t_lrn
+    log(f"learned gain: HyperLogLog {t_hll * 1e3:.2f} ms (workflow "
+        f"{st_hll['decision']['workflow']}), learned {t_lrn * 1e3:.2f} ms (workflow "
+        f"{st_lrn['decision']['workflow']}), factor {gain:.3f}")
+    assert gain > 1.0

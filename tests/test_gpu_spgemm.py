@@ -90,13 +90,20 @@ def test_fat_workflow(gpu, rows, bcols, vt, tol):
         f"timing {st['timing']['numeric']['fat']:.3f} ms")
 
 
-def test_rectangular_f32(gpu):
-    log("case: rectangular 2^14 x 2^17 poisson(8) f32, A*A^T")
-    A = gpu.CSR.rect_band(1 << 14, 1 << 17, 8.0, 0.3, 4096, 8, vtype=1, value_dist=1, seed=2)
+@pytest.mark.parametrize("log_rows", [14, 18])
+def test_rectangular_f32(gpu, log_rows):
+    """A*A^T with A 2^k x 2^(k+3), Poisson(8) entries per row, 30% of them
+    within +-4096 of row*8, FP32 N(0,1) values, seed 2; k = 18 is the
+    benchmark's rectangular configuration at full size (ESC<32> with hash
+    outliers, kernels/AccumulatorESC.cuh:129-183)."""
+    log(f"case: rectangular 2^{log_rows} x 2^{log_rows + 3} poisson(8) f32, A*A^T")
+    A = gpu.CSR.rect_band(1 << log_rows, 1 << (log_rows + 3), 8.0, 0.3, 4096, 8, vtype=1,
+                          value_dist=1, seed=2)
     B = A.transpose()
     C, st = _run(gpu, A, B)
     worst, plain = check_product(A, B, C, TOL32, "rect")
-    log(f"rect: workflow {st['decision']['workflow']} err {worst:.2e}")
+    log(f"rect 2^{log_rows}: workflow {st['decision']['workflow']}, nnz {C.nnz}, "
+        f"products {st['stats']['intermediate_product']}, err {worst:.2e}, max plain rel {plain:.2e}")
 
 
 @pytest.mark.parametrize("kind", ["er", "banded"])

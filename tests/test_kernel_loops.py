@@ -62,9 +62,24 @@ def loops(asm_text):
                 cur["parent"][cur["block"]] = cur["last_parent"]
             continue
         op = s.split()[0] if s else ""
-        if op == "s_barrier" or op.startswith("global_atomic_add"):
+        if (op == "s_barrier" or op.startswith("global_atomic_add") or op.startswith("ds_write")
+                or op.startswith("global_store")):
             cur["items"].append((op, cur["loop"].get(cur["block"])))
     return funcs
+
+
+def is_claim(its, i):
+    """A row claim publishes its result through LDS: the first store after
+    the atomic is an LDS write (an atomic that appends to a global list is
+    followed by a global store instead)."""
+    if not its[i][0].startswith("global_atomic_add"):
+        return False
+    for op, _ in its[i + 1:]:
+        if op.startswith("ds_write"):
+            return True
+        if op.startswith("global_store") or op == "s_barrier":
+            return False
+    return False
 
 
 def split_claims(funcs):
@@ -72,7 +87,7 @@ def split_claims(funcs):
     for name, f in funcs.items():
         its = f["items"]
         for i, (op, lp) in enumerate(its):
-            if not op.startswith("global_atomic_add"):
+            if not is_claim(its, i):
                 continue
             nxt = [x for x in its[i + 1:] if x[0] == "s_barrier"]
             if not nxt:
@@ -91,6 +106,7 @@ def test_parser_flags_a_split_loop():
     split = """_Zk:
 .LBB0_1:                               ; =>This Loop Header: Depth=1
 \tglobal_atomic_add v1, v2, v3, s[0:1] sc0
+\tds_write_b32 v4, v1 offset:512
 .LBB0_2:                               ;   Parent Loop BB0_1 Depth=1
                                         ; =>  This Loop Header: Depth=2
 \ts_barrier
@@ -102,6 +118,9 @@ def test_parser_flags_a_split_loop():
 """, "").replace(";   Parent Loop BB0_1 Depth=1", ";   in Loop: Header=BB0_1 Depth=1")
     assert len(split_claims(loops(split))) == 1
     assert split_claims(loops(sound)) == []
+    # an atomic append to a global list (followed by a global store) is not a claim
+    append = split.replace("\tds_write_b32 v4, v1 offset:512", "\tglobal_store_dword v4, v1, s[2:3]")
+    assert split_claims(loops(append)) == []
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
@@ -118,8 +137,8 @@ def test_persistent_loops_not_split():
     bad, claims = [], 0
     for src, text in asm.items():
         funcs = loops(text)
-        claims += sum(1 for f in funcs.values() for op, _ in f["items"]
-                      if op.startswith("global_atomic_add"))
+        claims += sum(1 for f in funcs.values() for i in range(len(f["items"]))
+                      if is_claim(f["items"], i))
         bad += [f"{src}: {b}" for b in split_claims(funcs)]
     assert claims > 5, "no row claims found: the assembly format changed"
     assert not bad, "\n".join(bad)

@@ -115,3 +115,17 @@ def test_symbolic_set_hash_and_rows():
     rrp, rcols, rvals, _ = ora.numeric_rows(A, A, rows)
     for i, r in enumerate(rows):
         assert np.array_equal(rcols[rrp[i]:rrp[i + 1]], cols[rp[r]:rp[r + 1]])
+
+
+def test_baseline_spgemm_matches_oracle():
+    """The CPU baseline's nnz and value sum equal the oracle's plain
+    Gustavson on the same rows (dense rows take the range scan, sparse rows
+    the sort)."""
+    import research_amd as ra
+    for A, r0, r1 in ((ra.CSR.rmat(12, 16 << 12, seed=1), 0, None),
+                      (ra.CSR.banded(3000, 65, seed=2), 100, 2900),
+                      (ra.CSR.erdos_renyi(5000, 6.0, seed=3), 7, 4000)):
+        n, cs = ora.baseline_spgemm(A, A, r0, r1, nthreads=4)
+        rp, cols, vals, _ = ora.numeric(A, A, r0, r1, compensated=0, with_cond=False)
+        assert n == len(cols)
+        assert abs(cs - vals.sum()) <= 1e-9 * max(1.0, np.abs(vals).sum())
