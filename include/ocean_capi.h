@@ -130,8 +130,8 @@ typedef struct oc_config {
   int32_t use_learned_binning;
   char learned_model_file[512];
   /* gfx950 kernel choice for rows beyond the LDS hash/dense kernels:
-   * 3 one-gather rank tiles (the symbolic pass stores the long rows'
-   * bitmaps), 2 rank-compressed tiles (default), 1 dense tiles, 0 cursor
+   * 3 one-gather rank tiles (default; the symbolic pass stores the long
+   * rows' bitmaps), 2 rank-compressed tiles, 1 dense tiles, 0 cursor
    * windows. */
   int32_t use_tiled_numeric;
   int32_t tile_log2_cols;  /* tile width 2^k columns (10..16), default 14 */
@@ -147,6 +147,16 @@ typedef struct oc_config {
    * at least this many times the sum of the predicted capacities, else the
    * precise workflow (default 1.5). */
   double learned_min_compression;
+  /* Per-bin shapes of the hash/dense kernels (bins 0..6; 0 = the gfx950
+   * default).  bin_concurrency: workgroups per CU the bin's tables are
+   * sized for (LDS per workgroup = 160 KB / concurrency), default
+   * {32,32,16,8,4,2,1}; bin_block: threads per workgroup (64..1024, a power
+   * of two), default {64,64,128,256,512,512,1024}.  The hash-table sizes,
+   * dense spans and bin thresholds follow from them, as BLOCK_SIZES and
+   * SHARED_MEM_PER_BLOCK fix them in the reference (include/Common.h:78-133);
+   * the autotuner searches them. */
+  int32_t bin_concurrency[7];
+  int32_t bin_block[7];
 } oc_config;
 
 /* Replaces: Config's default member initialisers (include/Utils.h:22-107). */
@@ -209,6 +219,24 @@ int oc_spgemm_plan(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
 int oc_spgemm_replay(oc_spgemm *h, void *stream);
 int oc_spgemm_capture(oc_spgemm *h);
 int oc_spgemm_graph_launch(oc_spgemm *h, void *stream);
+/* The whole precise path of a kept plan, from the analysis to the numeric
+ * kernels, recomputing structure and values from the current contents of
+ * the planned A and B (same dimensions and entry counts; the structure may
+ * change).  No reference counterpart: the reference reads bin sizes and
+ * totals back to the host between stages (kernels/SpGEMM.cuh:509-514,
+ * 611-615, 809-810, 1578-1580); here every bin kernel runs persistent over
+ * its device-side row range, so nothing is read back, allocated or set with
+ * a memset node.  oc_spgemm_full_replay runs it eagerly (synchronizing);
+ * oc_spgemm_capture_full records it into a HIP graph and
+ * oc_spgemm_graph_launch_full launches that graph.  dC keeps the planned
+ * capacity (the planned nnz).  After a run, oc_spgemm_full_status gives its
+ * check bits: 1 nnz(C) beyond the capacity, 2 an A row longer than the
+ * planned longest, 4 a symbolic table overflow, 8 a numeric recount; with
+ * any bit set, dC's contents are not the product. */
+int oc_spgemm_full_replay(oc_spgemm *h, void *stream);
+int oc_spgemm_capture_full(oc_spgemm *h);
+int oc_spgemm_graph_launch_full(oc_spgemm *h, void *stream);
+int oc_spgemm_full_status(oc_spgemm *h, uint32_t *status);
 
 /* ------------------------------------------------------ batched SpGEMM */
 /* No reference counterpart.  count independent products C_i = A_i * B_i
@@ -233,6 +261,14 @@ int oc_batch_run(oc_batch *b, void *stream);
 int oc_batch_plan(oc_batch *b, void *stream);
 int oc_batch_capture(oc_batch *b);
 int oc_batch_launch(oc_batch *b, void *stream);
+/* The whole path of every product in one graph (see oc_spgemm_capture_full):
+ * the inputs' structure and values are concatenated again and every
+ * product is recomputed from analysis to numeric; each input may change
+ * structure if it keeps its rows, columns and entry count.  Status bits as
+ * oc_spgemm_full_status. */
+int oc_batch_capture_full(oc_batch *b);
+int oc_batch_launch_full(oc_batch *b, void *stream);
+int oc_batch_full_status(oc_batch *b, uint32_t *status);
 /* The batched result: a view of C (u64 offsets; product i is rows
  * [row_base[i], row_base[i+1]) with columns shifted by col_base[i]) and
  * host copies of the count+1 bases (may be NULL). */

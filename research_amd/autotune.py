@@ -10,6 +10,10 @@ binned and which kernels and workgroup shapes take them:
   use_tiled_numeric, tile_log2_cols, tile_block   long-row kernel, tile width
                                                   and workgroup shape
   band_log2_cols                                  band-major long rows (0 off)
+  bin_concurrency[b], bin_block[b]                per-bin workgroups per CU
+                                                  (hence the LDS, hash-table
+                                                  size and dense span of bin
+                                                  b) and threads per workgroup
 
 Each candidate gets a handle of its own and one untimed run, which
 allocates the handle's memory and gives the output's structure hash; the
@@ -22,6 +26,11 @@ Config.load and the spgemm CLI read.
 
     python -m research_amd.autotune --workload rmat --log-n 18 --out tuned.json
     python -m research_amd.autotune --workload batched --count 256 --out tuned.json
+    python -m research_amd.autotune --workload heldout --out tuned.json
+
+`heldout` tunes on a set of R-MAT, Erdos-Renyi and banded products (the
+score is the sum of their times) and then times the R-MAT 2^20 benchmark
+product with the default and the tuned configuration.
 """
 from __future__ import annotations
 
@@ -37,11 +46,33 @@ from . import ocean
 SPACE = {
     "symbolic_expansion_coe": [1.25, 1.0, 1.5, 2.0, 3.0],
     "numeric_expansion_coe": [1.5, 1.25, 2.0],
-    "use_tiled_numeric": [2, 1],
+    "use_tiled_numeric": [3, 2, 1],
     "tile_log2_cols": [14, 13, 15],
     "tile_block": [1024, 512],
     "band_log2_cols": [0, 14, 13],
 }
+# gfx950 defaults of the per-bin shapes (0 in the config means these)
+BIN_CONCURRENCY = [32, 32, 16, 8, 4, 2, 1]
+BIN_BLOCK = [64, 64, 128, 256, 512, 512, 1024]
+for _b in range(7):
+    c0, k0 = BIN_CONCURRENCY[_b], BIN_BLOCK[_b]
+    # half or twice the workgroups per CU (twice / half the LDS each)
+    SPACE[("bin_concurrency", _b)] = [0] + [c for c in (c0 // 2, c0 * 2) if 1 <= c <= 32]
+    SPACE[("bin_block", _b)] = [0] + [k for k in (k0 // 2, k0 * 2) if 64 <= k <= 1024]
+
+
+def _config(cur: dict) -> dict:
+    """Config keys of a point of the space (per-bin knobs into arrays)."""
+    cfg = {k: v for k, v in cur.items() if isinstance(k, str)}
+    for name in ("bin_concurrency", "bin_block"):
+        arr = [cur.get((name, b), 0) for b in range(7)]
+        if any(arr):
+            cfg[name] = arr
+    return cfg
+
+
+def _knob_name(k) -> str:
+    return k if isinstance(k, str) else f"{k[0]}[{k[1]}]"
 
 
 class _Product:
@@ -82,6 +113,35 @@ class _Batched:
         return run
 
 
+class _Set:
+    """Several products; the score is the sum of their times and the
+    structure is the list of their hashes."""
+
+    def __init__(self, mats):
+        self.items = [_Product(M) for M in mats]
+
+    def session(self, cfg: dict):
+        runs = [w.session(cfg) for w in self.items]
+
+        def run(want_hash: bool):
+            out = [r(want_hash) for r in runs]
+            ms = sum(o[0] for o in out)
+            h = np.concatenate([o[1] for o in out]) if want_hash else None
+            return ms, h
+
+        return run
+
+
+def heldout_set():
+    """R-MAT, Erdos-Renyi and banded products, none of them the benchmark's
+    R-MAT 2^20."""
+    return [ocean.CSR.rmat(17, 16 << 17, vtype=0, seed=11),
+            ocean.CSR.rmat(16, 32 << 16, vtype=0, seed=12),
+            ocean.CSR.erdos_renyi(1 << 17, 24.0, vtype=0, seed=13),
+            ocean.CSR.banded(1 << 17, 33, vtype=0, seed=14),
+            ocean.CSR.rmat(18, 8 << 18, vtype=1, seed=15)]
+
+
 def _score(w, cfg, reps):
     run = w.session(cfg)
     _, h = run(True)  # untimed: allocates the handle's memory
@@ -95,8 +155,8 @@ def tune(w, reps: int = 3, passes: int = 2, log=print, gain: float = 0.01) -> di
     and its output structure equals the default's.  Repeated scores of one
     configuration on a warmed handle agree to about 0.3%."""
     cur = {k: v[0] for k, v in SPACE.items()}
-    best_ms, ref_hash = _score(w, cur, reps)
-    history = [{"config": dict(cur), "ms": best_ms}]
+    best_ms, ref_hash = _score(w, _config(cur), reps)
+    history = [{"config": _config(cur), "ms": best_ms}]
     if log:
         log(f"[autotune] default: {best_ms:.2f} ms", flush=True)
     for p in range(passes):
@@ -105,29 +165,30 @@ def tune(w, reps: int = 3, passes: int = 2, log=print, gain: float = 0.01) -> di
             for v in values:
                 if v == cur[knob]:
                     continue
-                cand = dict(cur, **{knob: v})
+                cand = dict(cur)
+                cand[knob] = v
                 try:
-                    ms, h = _score(w, cand, reps)
+                    ms, h = _score(w, _config(cand), reps)
                 except ocean.OceanError as e:
-                    history.append({"config": cand, "error": str(e)})
+                    history.append({"config": _config(cand), "error": str(e)})
                     continue
                 ok = np.array_equal(h, ref_hash)
-                history.append({"config": cand, "ms": ms, "same_structure": bool(ok)})
+                history.append({"config": _config(cand), "ms": ms, "same_structure": bool(ok)})
                 if log:
-                    log(f"[autotune] pass {p} {knob}={v}: {ms:.2f} ms"
+                    log(f"[autotune] pass {p} {_knob_name(knob)}={v}: {ms:.2f} ms"
                         f"{'' if ok else ' (structure differs: rejected)'}", flush=True)
                 if ok and ms < best_ms * (1.0 - gain):
                     best_ms, cur, changed = ms, cand, True
         if not changed:
             break
-    return {"config": cur, "ms": best_ms, "default_ms": history[0]["ms"],
+    return {"config": _config(cur), "ms": best_ms, "default_ms": history[0]["ms"],
             "speedup": round(history[0]["ms"] / best_ms, 3), "evaluations": len(history),
             "history": history}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m research_amd.autotune")
-    ap.add_argument("--workload", choices=["rmat", "batched"], default="rmat")
+    ap.add_argument("--workload", choices=["rmat", "batched", "heldout"], default="rmat")
     ap.add_argument("--log-n", type=int, default=18)
     ap.add_argument("--degree", type=int, default=16)
     ap.add_argument("--count", type=int, default=256)
@@ -138,12 +199,24 @@ def main(argv=None):
     if a.workload == "rmat":
         w = _Product(ocean.CSR.rmat(a.log_n, a.degree << a.log_n, vtype=0, seed=1))
         desc = f"R-MAT n=2^{a.log_n}, degree {a.degree}, f64"
+    elif a.workload == "heldout":
+        w = _Set(heldout_set())
+        desc = ("held-out set: R-MAT 2^17 deg 16, R-MAT 2^16 deg 32, ER 2^17 deg 24, "
+                "banded 2^17 bw 33 (f64), R-MAT 2^18 deg 8 (f32)")
     else:
         from . import workloads
         w = _Batched(workloads.batched_products(ocean, a.count))
         desc = f"{a.count} batched products (ER / banded / R-MAT, f32)"
     res = tune(w, a.reps)
     res["workload"] = desc
+    if a.workload == "heldout":
+        # the benchmark product, which the tuning never saw
+        bench = _Product(ocean.CSR.rmat(20, 16 << 20, vtype=0, seed=1))
+        d_ms, d_h = _score(bench, _config({k: v[0] for k, v in SPACE.items()}), a.reps)
+        t_ms, t_h = _score(bench, res["config"], a.reps)
+        res["benchmark"] = {"product": "R-MAT 2^20 deg 16 f64", "default_ms": round(d_ms, 2),
+                            "tuned_ms": round(t_ms, 2), "speedup": round(d_ms / t_ms, 3),
+                            "same_structure": bool(np.array_equal(d_h, t_h))}
     res["seconds"] = round(time.time() - t0, 1)
     if a.out:
         with open(a.out, "w") as f:

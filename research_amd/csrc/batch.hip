@@ -91,6 +91,8 @@ __global__ void k_batch_extract(const uint64_t *__restrict__ rp, const uint32_t 
 }
 
 int spgemm_replay_async(oc_spgemm *h, hipStream_t s);
+int spgemm_full_replay_async(oc_spgemm *h, hipStream_t s);
+int spgemm_full_status(oc_spgemm *h, uint32_t *status);
 void spgemm_drop_plan(oc_spgemm *h);
 void spgemm_set_blocks(oc_spgemm *h, const uint32_t *b_block_base, uint64_t span,
                        const void *b_row_offsets);
@@ -113,12 +115,18 @@ struct oc_batch {
   hipStream_t own = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipGraph_t graph_full = nullptr;  // the whole path (oc_batch_capture_full)
+  hipGraphExec_t exec_full = nullptr;
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (exec_full) (void)hipGraphExecDestroy(exec_full);
+    if (graph_full) (void)hipGraphDestroy(graph_full);
     exec = nullptr;
     graph = nullptr;
+    exec_full = nullptr;
+    graph_full = nullptr;
   }
   void drop_result() {
     drop_graph();
@@ -284,6 +292,50 @@ extern "C" int oc_batch_capture(oc_batch *b) {
   b->graph = g;
   HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
   return OC_OK;
+}
+
+// The whole path in one graph: the inputs' structure and values are
+// concatenated again, then analysis, binning, symbolic and numeric run for
+// every product with bin sizes kept on the device (see
+// oc_spgemm_capture_full).  The inputs may change structure between
+// launches as long as each keeps its rows, columns and entry count.
+extern "C" int oc_batch_capture_full(oc_batch *b) {
+  OC_REQUIRE(b && b->planned, "oc_batch_capture_full: run oc_batch_plan first");
+  if (b->exec_full) (void)hipGraphExecDestroy(b->exec_full);
+  if (b->graph_full) (void)hipGraphDestroy(b->graph_full);
+  b->exec_full = nullptr;
+  b->graph_full = nullptr;
+  if (!b->own) HIP_TRY(hipStreamCreate(&b->own));
+  HIP_TRY(hipStreamBeginCapture(b->own, hipStreamCaptureModeRelaxed));
+  int rc = concat(b, b->dA, b->A, b->a_total_nnz, false, b->own);
+  if (rc == OC_OK && !b->b_is_a) rc = concat(b, b->dB, b->B, b->b_total_nnz, false, b->own);
+  if (rc == OC_OK) rc = spgemm_full_replay_async(b->h, b->own);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->own, &g);
+  if (rc != OC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_TRY(e);
+  b->graph_full = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec_full, g, nullptr, nullptr, 0));
+  return OC_OK;
+}
+
+extern "C" int oc_batch_launch_full(oc_batch *b, void *stream) {
+  OC_REQUIRE(b && b->exec_full, "oc_batch_launch_full: run oc_batch_capture_full first");
+  if (stream) {
+    HIP_TRY(hipGraphLaunch(b->exec_full, (hipStream_t)stream));
+  } else {
+    HIP_TRY(hipGraphLaunch(b->exec_full, b->own));
+    HIP_TRY(hipStreamSynchronize(b->own));
+  }
+  return OC_OK;
+}
+
+extern "C" int oc_batch_full_status(oc_batch *b, uint32_t *status) {
+  OC_REQUIRE(b && status, "oc_batch_full_status: bad arguments");
+  return spgemm_full_status(b->h, status);
 }
 
 extern "C" int oc_batch_launch(oc_batch *b, void *stream) {

@@ -11,6 +11,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "oc_internal.hpp"
 
@@ -58,7 +59,7 @@ extern "C" void oc_config_default(oc_config *c) {
   c->print_info = 0;
   c->print_memory = 0;
   c->use_learned_binning = 0;
-  c->use_tiled_numeric = 2;
+  c->use_tiled_numeric = 3;
   c->tile_log2_cols = 14;
   c->tile_block = 1024;
   c->tile_products_cap = 0;
@@ -66,15 +67,17 @@ extern "C" void oc_config_default(oc_config *c) {
   c->band_log2_cols = 0;
   c->band_min_products = 16384;
   c->learned_min_compression = 1.5;
+  for (int b = 0; b < 7; ++b) c->bin_concurrency[b] = c->bin_block[b] = 0;
 }
 
 namespace {
 
 struct JValue {
-  enum Kind { NUM, BOOL, STR, NUL, OTHER } kind = OTHER;
+  enum Kind { NUM, BOOL, STR, NUL, ARR, OTHER } kind = OTHER;
   double num = 0;
   bool b = false;
   std::string s;
+  std::vector<double> arr;  // ARR: a flat array of numbers
 };
 
 struct Parser {
@@ -132,7 +135,37 @@ struct Parser {
       v.kind = JValue::STR;
       return str(v.s);
     }
-    if (*p == '{' || *p == '[') {  // skip nested structure
+    if (*p == '[') {  // a flat array of numbers, else skipped
+      const char *start = p;
+      ++p;
+      ws();
+      v.arr.clear();
+      bool flat = true;
+      while (*p && *p != ']') {
+        char *end = nullptr;
+        const double x = std::strtod(p, &end);
+        if (end == p) {
+          flat = false;
+          break;
+        }
+        v.arr.push_back(x);
+        p = end;
+        ws();
+        if (*p == ',') {
+          ++p;
+          ws();
+        }
+      }
+      if (flat && *p == ']') {
+        ++p;
+        v.kind = JValue::ARR;
+        return true;
+      }
+      p = start;
+      v.kind = JValue::OTHER;
+      return skip_nested();
+    }
+    if (*p == '{') {  // skip nested structure
       v.kind = JValue::OTHER;
       return skip_nested();
     }
@@ -206,6 +239,11 @@ bool as_dbl(const JValue &v, double &out) {
   }
   return false;
 }
+bool as_ints(const JValue &v, int32_t *out, size_t n) {
+  if (v.kind != JValue::ARR || v.arr.size() != n) return false;
+  for (size_t i = 0; i < n; ++i) out[i] = (int32_t)v.arr[i];
+  return true;
+}
 bool as_str(const JValue &v, char *out, size_t cap) {
   if (v.kind != JValue::STR) return false;
   std::strncpy(out, v.s.c_str(), cap - 1);
@@ -237,6 +275,12 @@ extern "C" int oc_config_check(const oc_config *c) {
   OC_REQUIRE(c->tile_log2_cols >= 10 && c->tile_log2_cols <= 16, "tile_log2_cols must be 10..16");
   OC_REQUIRE(c->band_log2_cols == 0 || (c->band_log2_cols >= 10 && c->band_log2_cols <= 15),
              "band_log2_cols must be 0 or 10..15");
+  for (int b = 0; b < 7; ++b) {
+    const int32_t cc = c->bin_concurrency[b], bb = c->bin_block[b];
+    OC_REQUIRE(cc >= 0 && cc <= 64, "bin_concurrency[%d] must be 0 (default) or 1..64", b);
+    OC_REQUIRE(bb == 0 || bb == 64 || bb == 128 || bb == 256 || bb == 512 || bb == 1024,
+               "bin_block[%d] must be 0 (default) or 64, 128, 256, 512, 1024", b);
+  }
   return OC_OK;
 }
 
@@ -270,6 +314,7 @@ extern "C" int oc_config_from_json(oc_config *c, const char *text) {
 #define D(name, field) else if (!std::strcmp(k, name)) ok = as_dbl(v, c->field);
 #define S(name, field) \
   else if (!std::strcmp(k, name)) ok = as_str(v, c->field, sizeof(c->field));
+#define IA(name, field) else if (!std::strcmp(k, name)) ok = as_ints(v, c->field, 7);
     if (false) {
     }
     // src/Utils.cpp:9-36
@@ -309,6 +354,9 @@ extern "C" int oc_config_from_json(oc_config *c, const char *text) {
     I("band_log2_cols", band_log2_cols)
     I("band_min_products", band_min_products)
     D("learned_min_compression", learned_min_compression)
+    IA("bin_concurrency", bin_concurrency)
+    IA("bin_block", bin_block)
+#undef IA
 #undef B
 #undef I
 #undef D

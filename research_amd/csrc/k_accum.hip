@@ -285,8 +285,8 @@ constexpr uint32_t kDenseHeavy = 32;  // dense kernels: hdr[32..63], count hdr[4
 // reaches the limit, or whose probe sequence wraps, is listed as overflow
 // and recounted by the windowed kernel.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
-                                                 const uint32_t *__restrict__ rows,
+__device__ __forceinline__ void sym_hash_row(KArgs<float> a,
+                                                 uint32_t r,
                                                  uint32_t S, uint32_t limit,
                                                  uint32_t *__restrict__ nnz,
                                                  uint32_t *ovf_rows,
@@ -294,7 +294,6 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
   extern __shared__ uint32_t lds_u32[];
   uint32_t *hdr = lds_u32;
   uint32_t *tab = lds_u32 + 64;
-  const uint32_t r = rows[blockIdx.x];
   for (uint32_t i = threadIdx.x; i < S; i += BS) tab[i] = kEmpty;
   if (threadIdx.x < 4) hdr[threadIdx.x] = 0;
   __syncthreads();
@@ -371,6 +370,21 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
   }
 }
 
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a, const uint32_t *__restrict__ rows,
+    const uint32_t *__restrict__ bounds, uint32_t S, uint32_t limit, uint32_t *__restrict__ nnz, uint32_t *ovf_rows, uint32_t *ovf_count) {
+  if (!bounds) {
+    sym_hash_row<BS>(a, rows[blockIdx.x], S, limit, nnz, ovf_rows, ovf_count);
+    return;
+  }
+  // persistent: the rows [bounds[0], bounds[1]) of the list
+  const uint32_t b1 = bounds[1];
+  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
+    sym_hash_row<BS>(a, rows[i], S, limit, nnz, ovf_rows, ovf_count);
+    __syncthreads();  // the next row re-initialises the LDS
+  }
+}
+
 // ========================================================= symbolic dense
 // Exact row nnz from an LDS bitmap over [left, right]
 // (kernels/AccumulatorDense.cuh:43-126).
@@ -378,15 +392,14 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a,
 // n4 uint4 groups from column left & ~127, at bm + bmoff[r], for the
 // one-gather rank tiles of the numeric pass.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
-                                                  const uint32_t *__restrict__ rows,
+__device__ __forceinline__ void sym_dense_row(KArgs<float> a,
+                                                  uint32_t r,
                                                   uint32_t *__restrict__ nnz,
                                                   const uint64_t *__restrict__ bmoff,
                                                   uint4 *__restrict__ bm) {
   extern __shared__ uint32_t lds_u32[];
   uint32_t *hdr = lds_u32;
   uint32_t *bits = lds_u32 + 64;
-  const uint32_t r = rows[blockIdx.x];
   const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
   const uint32_t lo = a.left[r] & ~127u, hi = a.right[r];
   if (a1 == a0 || a.products[r] == 0 || hi < a.left[r]) {
@@ -433,6 +446,21 @@ __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a,
   if (threadIdx.x == 0) nnz[r] = hdr[0];
 }
 
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a, const uint32_t *__restrict__ rows,
+    const uint32_t *__restrict__ bounds, uint32_t *__restrict__ nnz, const uint64_t *__restrict__ bmoff, uint4 *__restrict__ bm) {
+  if (!bounds) {
+    sym_dense_row<BS>(a, rows[blockIdx.x], nnz, bmoff, bm);
+    return;
+  }
+  // persistent: the rows [bounds[0], bounds[1]) of the list
+  const uint32_t b1 = bounds[1];
+  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
+    sym_dense_row<BS>(a, rows[i], nnz, bmoff, bm);
+    __syncthreads();  // the next row re-initialises the LDS
+  }
+}
+
 // ========================================================== numeric hash
 // C row values in an LDS hash table, FP64 sums via LDS atomics
 // (kernels/AccumulatorHash.cuh:509-651), then compaction, LDS radix sort
@@ -466,8 +494,8 @@ __host__ __device__ inline HashLayout hash_layout(uint32_t S, uint32_t nwaves) {
 }
 
 template <int BS, typename T, bool EST>
-__global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
-                                                 const uint32_t *__restrict__ rows,
+__device__ __forceinline__ void num_hash_row(KArgs<T> a,
+                                                 uint32_t r,
                                                  HashLayout L, KOut<T> o,
                                                  int key_bits) {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -479,7 +507,6 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
   uint16_t *ci = reinterpret_cast<uint16_t *>(smem + L.off_ci);
   uint16_t *cnt = reinterpret_cast<uint16_t *>(smem + L.off_cnt);
   const uint32_t S = L.S;
-  const uint32_t r = rows[blockIdx.x];
   for (uint32_t i = threadIdx.x; i < S; i += BS) {
     keys[i] = kEmpty;
     vals[i] = 0.0;
@@ -592,18 +619,33 @@ __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a,
   if (o.nnz && threadIdx.x == 0) o.nnz[r] = n;
 }
 
+template <int BS, typename T, bool EST>
+__global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a, const uint32_t *__restrict__ rows,
+    const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err, HashLayout L, KOut<T> o, int key_bits) {
+  if (err && *err) return;  // the capacity check failed: write nothing
+  if (!bounds) {
+    num_hash_row<BS, T, EST>(a, rows[blockIdx.x], L, o, key_bits);
+    return;
+  }
+  // persistent: the rows [bounds[0], bounds[1]) of the list
+  const uint32_t b1 = bounds[1];
+  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
+    num_hash_row<BS, T, EST>(a, rows[i], L, o, key_bits);
+    __syncthreads();  // the next row re-initialises the LDS
+  }
+}
+
 // ========================================================= numeric dense
 // C row values over the LDS span [left, right] (FP64 sums + bitmap), then
 // an ordered compaction (kernels/AccumulatorDense.cuh:132-346).  EST: rows
 // whose count exceeds cap[r] are listed as overflow
 // (kernels/AccumulatorDense.cuh:327-339).
 template <int BS, typename T, bool EST>
-__global__ __launch_bounds__(BS) void k_num_dense(KArgs<T> a,
-                                                  const uint32_t *__restrict__ rows,
+__device__ __forceinline__ void num_dense_row(KArgs<T> a,
+                                                  uint32_t r,
                                                   KOut<T> o) {
   extern __shared__ __align__(16) uint8_t smem[];
   uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
-  const uint32_t r = rows[blockIdx.x];
   const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
   const uint32_t lo = a.left[r] & ~31u, hi = a.right[r];
   if (a1 == a0 || a.products[r] == 0 || hi < a.left[r]) {
@@ -665,6 +707,22 @@ __global__ __launch_bounds__(BS) void k_num_dense(KArgs<T> a,
   if (o.nnz && threadIdx.x == 0) o.nnz[r] = total;
 }
 
+template <int BS, typename T, bool EST>
+__global__ __launch_bounds__(BS) void k_num_dense(KArgs<T> a, const uint32_t *__restrict__ rows,
+    const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err, KOut<T> o) {
+  if (err && *err) return;  // the capacity check failed: write nothing
+  if (!bounds) {
+    num_dense_row<BS, T, EST>(a, rows[blockIdx.x], o);
+    return;
+  }
+  // persistent: the rows [bounds[0], bounds[1]) of the list
+  const uint32_t b1 = bounds[1];
+  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
+    num_dense_row<BS, T, EST>(a, rows[i], o);
+    __syncthreads();  // the next row re-initialises the LDS
+  }
+}
+
 // ============================================================== windowed
 // Persistent kernels (one 1024-thread workgroup per CU) over the rows of
 // a bin, taken from an atomic counter.  The row's column span is swept in
@@ -681,7 +739,13 @@ template <typename T, bool SYM>
 __global__ __launch_bounds__(kWinBlock) void k_window(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows,
     uint32_t *counter, uint32_t W, KOut<T> o, uint32_t *__restrict__ nnz_sym,
-    uint32_t *__restrict__ scratch, uint32_t max_nA) {
+    uint32_t *__restrict__ scratch, uint32_t max_nA, const uint32_t *__restrict__ bounds,
+    const uint32_t *__restrict__ err) {
+  if (err && *err) return;  // a capacity check of the run failed
+  if (bounds) {             // rows [bounds[0], bounds[1]) of the list
+    rows += bounds[0];
+    nrows = bounds[1] - bounds[0];
+  }
   extern __shared__ __align__(16) uint8_t smem[];
   uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
   uint32_t *lcur = reinterpret_cast<uint32_t *>(smem + 256);
@@ -997,7 +1061,13 @@ __global__ __launch_bounds__(BS, 4) void k_num_rank(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
     uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1,
     const uint32_t *__restrict__ tsw, const uint32_t *__restrict__ bbase, uint32_t cap,
-    uint32_t pcap, KOut<T> o) {
+    uint32_t pcap, KOut<T> o, const uint32_t *__restrict__ bounds,
+    const uint32_t *__restrict__ err) {
+  if (err && *err) return;  // a capacity check of the run failed
+  if (bounds) {             // rows [bounds[0], bounds[1]) of the list
+    rows += bounds[0];
+    nrows = bounds[1] - bounds[0];
+  }
   using S = RankShape<BS>;
   constexpr uint32_t E = S::E, U = S::U, V = U / 8;
   extern __shared__ __align__(16) uint8_t smem[];
@@ -1380,8 +1450,9 @@ int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row) {
 
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
                     const uint32_t *ts, uint32_t T1, const uint32_t *tsw, const uint32_t *bbase,
-                    uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s) {
-  if (nrows == 0) return OC_OK;
+                    uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s,
+                    const RowSel &sel) {
+  if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
   uint32_t *counter = (uint32_t *)scratch;
   OC_TRY(zero_u32(counter, 1, s));
@@ -1391,8 +1462,8 @@ int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint
     const uint32_t per_cu = RankShape<B>::PER_CU;                                   \
     const size_t budget =                                                           \
         std::min<size_t>(t.lds_max_block, t.lds_per_cu / per_cu) & ~255ull;         \
-    const uint32_t grid =                                                           \
-        std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu) * per_cu);        \
+    const uint32_t full = std::max<uint32_t>(1, t.num_cu) * per_cu;                 \
+    const uint32_t grid = sel.bounds ? full : std::min<uint32_t>(nrows, full);      \
     const size_t fixed = RankShape<B>::lds_fixed();                                 \
     if (fixed + 64 * 12 > budget) {                                                 \
       set_error("rank-tile kernel does not fit the LDS of this device");            \
@@ -1403,7 +1474,7 @@ int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint
     OC_TRY(set_lds(k_num_rank<T, B>, lds));                                         \
     hipLaunchKernelGGL((k_num_rank<T, B>), dim3(grid), dim3(B), lds, s, kargs<T>(a), \
                        rows, nrows, counter, logT, ts, T1, tsw, bbase, cap, pcap,         \
-                       kout<T>(o));                                                          \
+                       kout<T>(o), sel.bounds, sel.err);                                     \
   } while (0)
   if (a.A.vtype == OC_F64) {
     if (bs == 256) GO(double, 256);
@@ -1491,8 +1562,8 @@ static size_t window_lds(uint32_t W, bool sym) {
 template <typename T, bool SYM>
 static int window_go(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                      uint32_t max_a_nnz, const NumOut &o, uint32_t *nnz_sym,
-                     void *scratch, hipStream_t s) {
-  if (nrows == 0) return OC_OK;
+                     void *scratch, hipStream_t s, const RowSel &sel) {
+  if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
   const uint32_t W = SYM ? t.win_sym_cols : t.win_num_cols;
   const size_t lds = window_lds(W, SYM);
@@ -1504,51 +1575,60 @@ static int window_go(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
   uint32_t *counter = (uint32_t *)scratch;
   uint32_t *cur = counter + 64;
   OC_TRY(zero_u32(counter, 1, s));
-  const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu));
+  const uint32_t grid = sel.bounds ? std::max<uint32_t>(1, t.num_cu)
+                                  : std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu));
   KOut<T> ko = kout<T>(o);
   hipLaunchKernelGGL((k_window<T, SYM>), dim3(grid), dim3(kWinBlock), lds, s,
                      kargs<T>(a), rows, nrows, counter, W, ko, nnz_sym, cur,
-                     max_a_nnz);
+                     max_a_nnz, sel.bounds, sel.err);
   LAUNCH_CHECK();
   return OC_OK;
 }
 
 int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                       uint32_t max_a_nnz, uint32_t *nnz, void *scratch,
-                      hipStream_t s) {
+                      hipStream_t s, const RowSel &sel) {
   NumOut o;
   std::memset(&o, 0, sizeof(o));
-  return window_go<float, true>(a, rows, nrows, max_a_nnz, o, nnz, scratch, s);
+  return window_go<float, true>(a, rows, nrows, max_a_nnz, o, nnz, scratch, s, sel);
 }
 
 int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                       uint32_t max_a_nnz, const NumOut &o, void *scratch,
-                      hipStream_t s) {
+                      hipStream_t s, const RowSel &sel) {
   if (a.A.vtype == OC_F64)
-    return window_go<double, false>(a, rows, nrows, max_a_nnz, o, nullptr, scratch, s);
-  return window_go<float, false>(a, rows, nrows, max_a_nnz, o, nullptr, scratch, s);
+    return window_go<double, false>(a, rows, nrows, max_a_nnz, o, nullptr, scratch, s, sel);
+  return window_go<float, false>(a, rows, nrows, max_a_nnz, o, nullptr, scratch, s, sel);
 }
 
 // ------------------------------------------------------------ launchers
 // Block sizes per bin (gfx950): 64, 64, 128, 256, 512, 512, 1024.
-#define BIN_DISPATCH(bin, MACRO)          \
-  switch (bin) {                          \
-    case 0: MACRO(64); break;             \
-    case 1: MACRO(64); break;             \
-    case 2: MACRO(128); break;            \
-    case 3: MACRO(256); break;            \
-    case 4: MACRO(512); break;            \
-    case 5: MACRO(512); break;            \
-    case 6: MACRO(1024); break;           \
-    default:                              \
-      set_error("bad bin %d", bin);       \
-      return OC_ERR_INVALID;              \
+// the bin's workgroup size (tables().block, default {64,64,128,256,512,
+// 512,1024}) picks the kernel instance
+#define BIN_DISPATCH(bin, MACRO)                                   \
+  switch ((bin) >= 0 && (bin) < 7 ? tables().block[(bin)] : 0u) { \
+    case 64: MACRO(64); break;                                     \
+    case 128: MACRO(128); break;                                   \
+    case 256: MACRO(256); break;                                   \
+    case 512: MACRO(512); break;                                   \
+    case 1024: MACRO(1024); break;                                 \
+    default:                                                       \
+      set_error("bad bin %d", bin);                                \
+      return OC_ERR_INVALID;                                       \
   }
+
+// Workgroups of a persistent launch: as many as fit on the chip at once.
+static uint32_t persistent_grid(uint32_t bs, size_t lds) {
+  const Tables &t = tables();
+  const uint32_t by_lds = lds ? (uint32_t)(t.lds_per_cu / lds) : 64u;
+  const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(2048 / bs, by_lds));
+  return std::max<uint32_t>(1, t.num_cu) * per_cu;
+}
 
 int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
                     uint32_t nrows, uint32_t limit, uint32_t *nnz,
-                    uint32_t *ovf_rows, uint32_t *ovf_count, hipStream_t s) {
-  if (nrows == 0) return OC_OK;
+                    uint32_t *ovf_rows, uint32_t *ovf_count, hipStream_t s, const RowSel &sel) {
+  if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
   const uint32_t S = t.hash_sym[bin];
   const size_t lds = 256 + (size_t)S * 4;
@@ -1556,8 +1636,10 @@ int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
 #define GO(BS)                                                                \
   do {                                                                        \
     OC_TRY(set_lds(k_sym_hash<BS>, lds));                                     \
-    hipLaunchKernelGGL(k_sym_hash<BS>, dim3(nrows), dim3(BS), lds, s, k, rows, \
-                       S, std::min(limit, S), nnz, ovf_rows, ovf_count);      \
+    const uint32_t grid = sel.bounds ? persistent_grid(BS, lds) : nrows;      \
+    hipLaunchKernelGGL(k_sym_hash<BS>, dim3(grid), dim3(BS), lds, s, k, rows,  \
+                       sel.bounds, S, std::min(limit, S), nnz, ovf_rows,      \
+                       ovf_count);                                            \
   } while (0)
   BIN_DISPATCH(bin, GO)
 #undef GO
@@ -1567,17 +1649,21 @@ int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
 
 int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
                      uint32_t nrows, uint32_t *nnz, hipStream_t s, const uint64_t *bmoff,
-                     void *bm) {
-  if (nrows == 0) return OC_OK;
+                     void *bm, const RowSel &sel) {
+  if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
+  // the widest bin's 1024-thread rows: flat product enumeration (k_rank.hip)
+  if (dense_bin == 6 && t.block[6] == 1024)
+    return launch_sym_flat(t.dense_sym[6], a, rows, nrows, nnz, s, bmoff, bm, sel);
   // whole 16-byte groups from column left & ~127 over the widest span
   const size_t lds = 256 + ((size_t)(t.dense_sym[dense_bin] + 126) / 128 + 1) * 16;
   KArgs<float> k = kargs<float>(a);
 #define GO(BS)                                                                 \
   do {                                                                         \
     OC_TRY(set_lds(k_sym_dense<BS>, lds));                                     \
-    hipLaunchKernelGGL(k_sym_dense<BS>, dim3(nrows), dim3(BS), lds, s, k, rows, \
-                       nnz, bmoff, (uint4 *)bm);                               \
+    const uint32_t grid = sel.bounds ? persistent_grid(BS, lds) : nrows;       \
+    hipLaunchKernelGGL(k_sym_dense<BS>, dim3(grid), dim3(BS), lds, s, k, rows,  \
+                       sel.bounds, nnz, bmoff, (uint4 *)bm);                   \
   } while (0)
   BIN_DISPATCH(dense_bin, GO)
 #undef GO
@@ -1586,8 +1672,8 @@ int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
 }
 
 int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
-                    uint32_t nrows, const NumOut &o, hipStream_t s) {
-  if (nrows == 0) return OC_OK;
+                    uint32_t nrows, const NumOut &o, hipStream_t s, const RowSel &sel) {
+  if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
   const HashLayout L = hash_layout(t.hash_num[bin], t.block[bin] / 64);
   if (L.bytes > t.lds[bin]) {
@@ -1599,8 +1685,10 @@ int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
 #define GO3(BS, T, E)                                                          \
   do {                                                                         \
     OC_TRY(set_lds(k_num_hash<BS, T, E>, L.bytes));                            \
-    hipLaunchKernelGGL((k_num_hash<BS, T, E>), dim3(nrows), dim3(BS), L.bytes, \
-                       s, kargs<T>(a), rows, L, kout<T>(o), kb);               \
+    const uint32_t grid = sel.bounds ? persistent_grid(BS, L.bytes) : nrows;   \
+    hipLaunchKernelGGL((k_num_hash<BS, T, E>), dim3(grid), dim3(BS), L.bytes,  \
+                       s, kargs<T>(a), rows, sel.bounds, sel.err, L,           \
+                       kout<T>(o), kb);                                        \
   } while (0)
 #define GO(BS)                                            \
   do {                                                    \
@@ -1620,8 +1708,8 @@ int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
 }
 
 int launch_num_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
-                     uint32_t nrows, const NumOut &o, hipStream_t s) {
-  if (nrows == 0) return OC_OK;
+                     uint32_t nrows, const NumOut &o, hipStream_t s, const RowSel &sel) {
+  if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
   const uint32_t cap = t.dense_num[dense_bin];
   const uint32_t nwords = cap / 32 + 2;
@@ -1634,8 +1722,9 @@ int launch_num_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
 #define GO3(BS, T, E)                                                           \
   do {                                                                          \
     OC_TRY(set_lds(k_num_dense<BS, T, E>, lds));                                \
-    hipLaunchKernelGGL((k_num_dense<BS, T, E>), dim3(nrows), dim3(BS), lds, s,  \
-                       kargs<T>(a), rows, kout<T>(o));                          \
+    const uint32_t grid = sel.bounds ? persistent_grid(BS, lds) : nrows;        \
+    hipLaunchKernelGGL((k_num_dense<BS, T, E>), dim3(grid), dim3(BS), lds, s,   \
+                       kargs<T>(a), rows, sel.bounds, sel.err, kout<T>(o));     \
   } while (0)
 #define GO(BS)                                            \
   do {                                                    \

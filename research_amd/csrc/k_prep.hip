@@ -30,37 +30,31 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) {
 // ------------------------------------------------------------------ tables
 static Tables g_tab;
 static bool g_tab_ready = false;
+static uint32_t g_lds_cu = 163840, g_lds_blk = 163840, g_ncu = 256;
+// tables() on this thread while a TablesScope lives (per-handle shapes)
+static thread_local const Tables *g_override = nullptr;
 
-int tables_init() {
-  if (g_tab_ready) return OC_OK;
-  uint32_t lds_cu = 163840, lds_blk = 163840, ncu = 256;
-  int dev = 0;
-  hipDeviceProp_t p;
-  if (hipGetDevice(&dev) == hipSuccess &&
-      hipGetDeviceProperties(&p, dev) == hipSuccess) {
-    if (p.maxSharedMemoryPerMultiProcessor > 0)
-      lds_cu = (uint32_t)std::min<size_t>(p.maxSharedMemoryPerMultiProcessor, 163840);
-    if (p.sharedMemPerBlock > 0)
-      lds_blk = (uint32_t)std::min<size_t>(p.sharedMemPerBlock, 163840);
-    if (p.multiProcessorCount > 0) ncu = (uint32_t)p.multiProcessorCount;
-  } else {
-    (void)hipGetLastError();
-  }
-  static const uint32_t conc[7] = {32, 32, 16, 8, 4, 2, 1};
-  static const uint32_t block[7] = {64, 64, 128, 256, 512, 512, 1024};
+// Bin shapes (workgroups per CU and threads per workgroup) -> LDS per
+// workgroup, table sizes and dense spans.
+static Tables derive_tables(const int32_t *conc_in, const int32_t *block_in) {
+  static const uint32_t conc0[7] = {32, 32, 16, 8, 4, 2, 1};
+  static const uint32_t block0[7] = {64, 64, 128, 256, 512, 512, 1024};
   Tables t;
   std::memset(&t, 0, sizeof(t));
-  t.lds_per_cu = lds_cu;
-  t.lds_max_block = lds_blk;
-  t.num_cu = ncu;
+  t.lds_per_cu = g_lds_cu;
+  t.lds_max_block = g_lds_blk;
+  t.num_cu = g_ncu;
   for (int b = 0; b < 7; ++b) {
-    uint32_t l = std::min(lds_cu / conc[b], lds_blk) & ~255u;
-    uint32_t nw = block[b] / 64;
-    t.block[b] = block[b];
+    const uint32_t conc = conc_in && conc_in[b] > 0 ? (uint32_t)conc_in[b] : conc0[b];
+    const uint32_t block = block_in && block_in[b] > 0 ? (uint32_t)block_in[b] : block0[b];
+    uint32_t l = std::min(g_lds_cu / conc, g_lds_blk) & ~255u;
+    uint32_t nw = block / 64;
+    t.block[b] = block;
     t.lds[b] = l;
     t.hash_sym[b] = ((l - 256) / 4) & ~7u;
-    // dense bitmap over a span (plus one word of alignment slack)
-    t.dense_sym[b] = b == 0 ? 0 : (l - 512) * 8;
+    // dense bitmap over a span (plus one word of alignment slack); the
+    // widest bin's flat kernel also keeps 9216 B of segments and scan slots
+    t.dense_sym[b] = b == 0 ? 0 : (l - 512 - (b == 6 ? 9216u : 0u)) * 8;
     // numeric hash: keys (4 B) + FP64 sums (8 B) + sort buffers (4 B per
     // slot) + 1280 B header/digit totals + 512 B of radix counters per wave
     t.hash_num[b] = b == 0 ? 0 : ((l - 1312 - nw * 512) / 16) & ~7u;
@@ -76,15 +70,53 @@ int tables_init() {
   uint32_t ws = 1u << 24;
   while (ws > 32768 && (size_t)ws / 8 + 4096 * 4 + 1024 > t.lds[6]) ws >>= 1;
   t.win_sym_cols = ws;
-  g_tab = t;
+  return t;
+}
+
+int tables_init() {
+  if (g_tab_ready) return OC_OK;
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipGetDeviceProperties(&p, dev) == hipSuccess) {
+    if (p.maxSharedMemoryPerMultiProcessor > 0)
+      g_lds_cu = (uint32_t)std::min<size_t>(p.maxSharedMemoryPerMultiProcessor, 163840);
+    if (p.sharedMemPerBlock > 0)
+      g_lds_blk = (uint32_t)std::min<size_t>(p.sharedMemPerBlock, 163840);
+    if (p.multiProcessorCount > 0) g_ncu = (uint32_t)p.multiProcessorCount;
+  } else {
+    (void)hipGetLastError();
+  }
+  g_tab = derive_tables(nullptr, nullptr);
   g_tab_ready = true;
   return OC_OK;
 }
 
 const Tables &tables() {
+  if (g_override) return *g_override;
   tables_init();
   return g_tab;
 }
+
+int tables_for(const int32_t *conc, const int32_t *block, Tables *out) {
+  tables_init();
+  Tables t = derive_tables(conc, block);
+  for (int b = 0; b < 7; ++b) {
+    // every bin's numeric hash layout and dense span must fit its LDS
+    const uint32_t nw = t.block[b] / 64;
+    OC_REQUIRE(t.lds[b] >= 1312 + nw * 512 + 64 * 16 && t.lds[b] >= 2048,
+               "bin %d: %u bytes of LDS per workgroup cannot hold the tables of a "
+               "%u-thread workgroup (lower bin_concurrency or bin_block)",
+               b, t.lds[b], t.block[b]);
+  }
+  *out = t;
+  return OC_OK;
+}
+
+TablesScope::TablesScope(const Tables *t) : prev(g_override) {
+  if (t) g_override = t;
+}
+TablesScope::~TablesScope() { g_override = prev; }
 
 // ---------------------------------------------------------------- analysis
 // Per A row: products = sum of referenced B row lengths, longest B row, the
@@ -246,6 +278,30 @@ __global__ void k_zero3(uint32_t *a, uint32_t *b, uint32_t *c) {
     if (b) *b = 0;
     if (c) *c = 0;
   }
+}
+
+// Checks of a whole-path replay on the device: err |= bit when *x > limit.
+__global__ void k_check_gt(const void *x, int is64, uint64_t limit, uint32_t bit,
+                           uint32_t *err) {
+  if (threadIdx.x == 0) {
+    const uint64_t v = is64 ? *(const uint64_t *)x : (uint64_t) * (const uint32_t *)x;
+    if (v > limit) atomicOr(err, bit);
+  }
+}
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+int launch_check_gt(const void *x, int is64, uint64_t limit, uint32_t bit, uint32_t *err,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_check_gt, dim3(1), dim3(64), 0, s, x, is64, limit, bit, err);
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+int launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, p, n, v);
+  LAUNCH_CHECK();
+  return OC_OK;
 }
 
 size_t analysis_temp_bytes(uint64_t a_rows, uint64_t a_nnz, uint64_t b_rows) {

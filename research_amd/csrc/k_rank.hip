@@ -23,22 +23,31 @@
 namespace oc {
 
 constexpr uint32_t kR1Block = 1024;
-constexpr uint32_t kR1Waves = kR1Block / kWave;
-constexpr uint32_t kR1MaxLog = 17;                       // widest tile
-constexpr uint32_t kR1Words = (1u << kR1MaxLog) / 32;    // its bitmap words
-constexpr uint32_t kR1U = 8;                             // 64-position blocks per step
+constexpr uint32_t kR1U = 8;  // 64-position blocks per step
 
-// LDS bytes besides the cap slots (12 B each: FP64 sum + column).
-static size_t r1_lds_fixed() {
-  return 1024 + (size_t)kR1Block * 8 + (size_t)kR1Words * 8 + (size_t)kR1Block * 4 +
-         (size_t)(kR1Block + 4) * 4;
-}
+// Shape of a one-gather workgroup of BS threads: one A entry per thread
+// per chunk, widest tile 2^MAXLOG columns (1024 threads take a whole CU's
+// LDS; 512 threads half, two workgroups per CU, so one's gathers overlap
+// the other's scans and write-outs).
+template <int BS>
+struct R1Shape {
+  static constexpr uint32_t NW = BS / kWave;
+  static constexpr uint32_t MAXLOG = BS >= 1024 ? 17 : 16;
+  static constexpr uint32_t WORDS = (1u << MAXLOG) / 32;
+  static constexpr uint32_t PER_CU = 1024 / BS;
+  // LDS bytes besides the cap slots (12 B each: FP64 sum + column)
+  static size_t lds_fixed() {
+    return 1024 + (size_t)BS * 8 + (size_t)WORDS * 8 + (size_t)BS * 4 + (size_t)(BS + 4) * 4;
+  }
+  static_assert(BS == 512 || BS == 1024, "one-gather workgroups of 512 or 1024 threads");
+};
 
-// Exclusive block sums of three 32-bit values at once (1024 threads):
-// DPP wave scans, one LDS slot per wave and value, one barrier.  `scr`
+// Exclusive block sums of three 32-bit values at once (NW waves, at most
+// 16): DPP wave scans, one LDS slot per wave and value, one barrier.  `scr`
 // holds 64 entries; consecutive calls must alternate between two scratch
 // areas (the barrier of the next call orders this call's reads before the
 // call after it writes the same area again).
+template <uint32_t NW>
 __device__ __forceinline__ void block_excl_sum3(uint32_t x, uint32_t y, uint32_t z,
                                                 uint32_t *scr, uint32_t &ex, uint32_t &ey,
                                                 uint32_t &ez, uint32_t &tx, uint32_t &ty,
@@ -55,7 +64,8 @@ __device__ __forceinline__ void block_excl_sum3(uint32_t x, uint32_t y, uint32_t
     scr[32 + wid] = iz;
   }
   __syncthreads();
-  const bool in = lane < (int)kR1Waves;
+  static_assert(NW <= 16, "at most 16 waves");
+  const bool in = lane < (int)NW;
   const uint32_t sx = in ? scr[lane] : 0u, sy = in ? scr[16 + lane] : 0u,
                  sz = in ? scr[32 + lane] : 0u;
   const uint32_t jx = wave_incl_scan(sx, 0u, add);
@@ -79,31 +89,37 @@ struct BoolC {
 // u32).  bm/bmoff: stored row bitmaps (uint4 groups from column
 // left & ~127; bmoff[r] = ~0 for rows without one); only used without
 // block bases.
-template <typename T>
-__global__ __launch_bounds__(kR1Block) void k_num_rank1(
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void k_num_rank1(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
     uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1,
     const uint32_t *__restrict__ tsw, const uint32_t *__restrict__ bbase,
     const uint4 *__restrict__ bm, const uint64_t *__restrict__ bmoff, uint32_t cap,
-    KOut<T> o) {
-  constexpr uint32_t BS = kR1Block, U = kR1U;
+    KOut<T> o, const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err) {
+  if (err && *err) return;  // a capacity check of the run failed
+  if (bounds) {             // rows [bounds[0], bounds[1]) of the list
+    rows += bounds[0];
+    nrows = bounds[1] - bounds[0];
+  }
+  using SH = R1Shape<BS>;
+  constexpr uint32_t U = kR1U, NW = SH::NW, MAXLOG = SH::MAXLOG, WORDS = SH::WORDS;
   extern __shared__ __align__(16) uint8_t smem[];
   uint32_t *scr = reinterpret_cast<uint32_t *>(smem);  // two 64-entry scan areas
   uint32_t *hdr = reinterpret_cast<uint32_t *>(smem + 512);
   double *cav = reinterpret_cast<double *>(smem + 1024);  // A value of segment j
   uint2 *bw = reinterpret_cast<uint2 *>(cav + BS);        // {bits, rank of bit 0}
-  double *acc = reinterpret_cast<double *>(bw + kR1Words);
+  double *acc = reinterpret_cast<double *>(bw + WORDS);
   uint32_t *cq = reinterpret_cast<uint32_t *>(acc + cap);  // B position = cq[j] + p
   uint32_t *cex = cq + BS;                                 // first position of segment j
   uint32_t *cmap = cex + BS + 4;                           // column of rank slot
   const uint32_t tid = threadIdx.x, lane = lane_id();
   const uint32_t wid = __builtin_amdgcn_readfirstlane(tid / kWave);
-  for (uint32_t i = tid; i < kR1Words; i += BS) bw[i] = uint2{0u, 0u};
+  for (uint32_t i = tid; i < WORDS; i += BS) bw[i] = uint2{0u, 0u};
   for (uint32_t i = tid; i < cap; i += BS) acc[i] = 0.0;
   uint32_t sp = 0;  // scan scratch area of the next block scan
   auto scan3 = [&](uint32_t x, uint32_t y, uint32_t z, uint32_t &ex, uint32_t &ey,
                    uint32_t &ez, uint32_t &tx, uint32_t &ty, uint32_t &tz) {
-    block_excl_sum3(x, y, z, scr + 64 * sp, ex, ey, ez, tx, ty, tz);
+    block_excl_sum3<NW>(x, y, z, scr + 64 * sp, ex, ey, ez, tx, ty, tz);
     sp ^= 1u;
   };
   __syncthreads();
@@ -125,7 +141,7 @@ __global__ __launch_bounds__(kR1Block) void k_num_rank1(
       uint64_t est = prod < span ? prod : span;
       if (o.nnz_hint) est = o.nnz_hint[r];
       uint32_t lw = logT;
-      while (lw < kR1MaxLog && (span >> lw) >= 1 &&
+      while (lw < MAXLOG && (span >> lw) >= 1 &&
              est * (1ull << (lw + 1)) <= (uint64_t)cap * 3 / 4 * span)
         ++lw;
       const uint32_t step = lw - logT;
@@ -147,7 +163,7 @@ __global__ __launch_bounds__(kR1Block) void k_num_rank1(
       const uint32_t w0 = tid * wpt;
       const bool has_w = w0 < nwords;
       const bool one_chunk = nA <= BS;
-      const bool wide = tsw != nullptr && lw == kR1MaxLog;
+      const bool wide = tsw != nullptr && lw == MAXLOG;
       auto tsv = [&](uint32_t k, uint32_t tt) -> uint32_t {
         if (wide) return tsw[(uint64_t)k * kWideStarts + min(kWideStarts - 1, tt)];
         return ts[(uint64_t)k * T1 + min(T1 - 1, tt << step)];
@@ -228,7 +244,7 @@ __global__ __launch_bounds__(kR1Block) void k_num_rank1(
       auto enumerate = [&](auto markc, uint64_t wbase, uint32_t P, uint32_t ne, uint32_t r0,
                            uint32_t r1) {
         constexpr bool MARK = decltype(markc)::value;
-        const uint32_t per = ((P + kR1Waves - 1) / kR1Waves + 63) & ~63u;
+        const uint32_t per = ((P + NW - 1) / NW + 63) & ~63u;
         const uint32_t beg = min(P, wid * per), end = min(P, beg + per);
         if (beg >= end) return;
         // owner of position beg: the last segment starting at or before it
@@ -324,10 +340,12 @@ __global__ __launch_bounds__(kR1Block) void k_num_rank1(
         ts_cur = tsv(my_k, t_begin);
         ts_end = tsv(my_k, t_begin + 1);
       }
+      // stored words of the next tile are loaded while the current one is
+      // summed (the first tile's here)
+      uint32_t wv[4];
+      load_words((uint64_t)base_r + ((uint64_t)t_begin << lw), wv);
       for (uint32_t t = t_begin; t <= t_end; ++t) {
         const uint64_t wbase = (uint64_t)base_r + ((uint64_t)t << lw);
-        uint32_t wv[4];
-        load_words(wbase, wv);
         const uint32_t pc = __popc(wv[0]) + __popc(wv[1]) + __popc(wv[2]) + __popc(wv[3]);
         uint32_t P = 0, ne = 0, m = 0;
         if (one_chunk) {
@@ -341,7 +359,10 @@ __global__ __launch_bounds__(kR1Block) void k_num_rank1(
             cav[ej] = my_av;
           }
           if (tid == 0) cex[ne] = P;
-          if (stored) write_words(wv, ew);
+          if (stored) {
+            write_words(wv, ew);
+            if (t + 1 <= t_end) load_words(wbase + (1ull << lw), wv);
+          }
           ts_cur = ts_end;
           ts_end = nxt;
           __syncthreads();
@@ -354,6 +375,7 @@ __global__ __launch_bounds__(kR1Block) void k_num_rank1(
           uint32_t ew, d1, d2, t1, t2;
           scan3(pc, 0u, 0u, ew, d1, d2, m, t1, t2);
           write_words(wv, ew);  // published by the first chunk's barrier
+          if (t + 1 <= t_end) load_words(wbase + (1ull << lw), wv);
         } else {
           for (uint32_t c0 = a0; c0 < a1; c0 += BS) {
             setup_chunk(c0, min(BS, a1 - c0), t, P, ne);
@@ -396,39 +418,209 @@ __global__ __launch_bounds__(kR1Block) void k_num_rank1(
 int launch_num_rank1(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
                      const uint32_t *ts, uint32_t T1, const uint32_t *tsw,
                      const uint32_t *bbase, const void *bm, const uint64_t *bmoff,
-                     const NumOut &o, void *scratch, hipStream_t s) {
-  if (nrows == 0) return OC_OK;
+                     const NumOut &o, void *scratch, hipStream_t s, const RowSel &sel,
+                     uint32_t bs) {
+  if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
-  if (logT > kR1MaxLog || logT < 10) {
-    set_error("one-gather rank tiles need a tile-start table of 2^10..2^17 columns");
+  if (bs != 512) bs = 1024;
+  if (logT > rank1_wide_log(bs) || logT < 10) {
+    set_error("one-gather rank tiles of %u threads need a tile-start table of 2^10..2^%u "
+              "columns", bs, rank1_wide_log(bs));
     return OC_ERR_INVALID;
   }
   uint32_t *counter = (uint32_t *)scratch;
   OC_TRY(zero_u32(counter, 1, s));
-  const size_t budget = std::min<size_t>(t.lds_max_block, t.lds_per_cu) & ~255ull;
-  const size_t fixed = r1_lds_fixed();
-  if (fixed + 64 * 12 > budget) {
-    set_error("one-gather rank-tile kernel does not fit the LDS of this device");
-    return OC_ERR_INVALID;
-  }
-  const uint32_t cap = (uint32_t)(((budget - fixed) / 12) & ~63ull);
-  const size_t lds = fixed + (size_t)cap * 12;
-  const uint32_t grid = std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu));
-#define GO(T)                                                                            \
-  do {                                                                                   \
-    OC_TRY(set_lds(k_num_rank1<T>, lds));                                                \
-    hipLaunchKernelGGL((k_num_rank1<T>), dim3(grid), dim3(kR1Block), lds, s, kargs<T>(a), \
-                       rows, nrows, counter, logT, ts, T1, tsw, bbase,                   \
-                       (const uint4 *)bm, bmoff, cap, kout<T>(o));                       \
+#define GO(T, B)                                                                          \
+  do {                                                                                    \
+    const size_t budget =                                                                 \
+        std::min<size_t>(t.lds_max_block, t.lds_per_cu / R1Shape<B>::PER_CU) & ~255ull;   \
+    const size_t fixed = R1Shape<B>::lds_fixed();                                         \
+    if (fixed + 64 * 12 > budget) {                                                       \
+      set_error("one-gather rank-tile kernel does not fit the LDS of this device");       \
+      return OC_ERR_INVALID;                                                              \
+    }                                                                                     \
+    const uint32_t cap = (uint32_t)(((budget - fixed) / 12) & ~63ull);                    \
+    const size_t lds = fixed + (size_t)cap * 12;                                          \
+    const uint32_t full = std::max<uint32_t>(1, t.num_cu) * R1Shape<B>::PER_CU;           \
+    const uint32_t grid = sel.bounds ? full : std::min<uint32_t>(nrows, full);            \
+    OC_TRY(set_lds(k_num_rank1<T, B>, lds));                                              \
+    hipLaunchKernelGGL((k_num_rank1<T, B>), dim3(grid), dim3(B), lds, s, kargs<T>(a),     \
+                       rows, nrows, counter, logT, ts, T1, tsw, bbase,                    \
+                       (const uint4 *)bm, bmoff, cap, kout<T>(o), sel.bounds, sel.err);   \
   } while (0)
-  if (a.A.vtype == OC_F64) GO(double);
-  else GO(float);
+  if (a.A.vtype == OC_F64) {
+    if (bs == 512) GO(double, 512);
+    else GO(double, 1024);
+  } else {
+    if (bs == 512) GO(float, 512);
+    else GO(float, 1024);
+  }
 #undef GO
   LAUNCH_CHECK();
   return OC_OK;
 }
 
-uint32_t rank1_wide_log() { return kR1MaxLog; }
+uint32_t rank1_wide_log(uint32_t bs) {
+  return bs == 512 ? R1Shape<512>::MAXLOG : R1Shape<1024>::MAXLOG;
+}
+
+// ---------------------------------------------------- flat dense symbolic
+// Exact row nnz from an LDS bitmap over [left & ~127, right] (the role of
+// denseSymbolicKernel, kernels/AccumulatorDense.cuh:43-126) for the widest
+// dense symbolic bin.  The row's products are enumerated flat, as in the
+// rank tiles: A entries in chunks of 1024, their (non-empty) B rows
+// compacted into segments, each wave a contiguous range of positions with
+// the ballot-window owners, so every lane marks the same number of
+// columns however uneven the B rows are (lane groups per A entry left
+// about half the lanes idle on R-MAT's long rows).  Rows with
+// bmoff[r] != ~0 store their bitmap as k_sym_dense does.
+constexpr uint32_t kFlatEntryBytes = 8704;  // cex[1028] + cq[1024] + pad
+
+template <int BS>
+__device__ __forceinline__ void sym_flat_row(KArgs<float> a, uint32_t r,
+                                             uint32_t *__restrict__ nnz,
+                                             const uint64_t *__restrict__ bmoff,
+                                             uint4 *__restrict__ bm) {
+  static_assert(BS == kR1Block, "the flat symbolic kernel runs 1024-thread workgroups");
+  constexpr uint32_t U = kR1U;
+  extern __shared__ __align__(16) uint8_t smem[];
+  uint32_t *scr = reinterpret_cast<uint32_t *>(smem);  // two 64-entry scan areas
+  uint32_t *cex = reinterpret_cast<uint32_t *>(smem + 512);
+  uint32_t *cq = cex + BS + 4;
+  uint32_t *bits = reinterpret_cast<uint32_t *>(smem + 512 + kFlatEntryBytes);
+  const uint32_t tid = threadIdx.x, lane = lane_id();
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
+  const uint32_t lo = a.left[r] & ~127u, hi = a.right[r];
+  if (a1 == a0 || a.products[r] == 0 || hi < a.left[r]) {
+    if (tid == 0) nnz[r] = 0;
+    return;
+  }
+  const uint64_t bo = bmoff ? bmoff[r] : ~0ull;
+  const uint32_t n4 = ((hi - lo) >> 7) + 1;
+  uint4 *bits4 = reinterpret_cast<uint4 *>(bits);
+  for (uint32_t i = tid; i < n4; i += BS) bits4[i] = uint4{0, 0, 0, 0};
+  uint32_t sp = 0;
+  for (uint32_t c0 = a0; c0 < a1; c0 += BS) {
+    const uint32_t nc = min(BS, a1 - c0);
+    uint32_t len = 0, st = 0;
+    if (tid < nc) {
+      const uint32_t k = a.a_ci[c0 + tid];
+      st = a.b_rp[k];
+      len = a.b_rp[k + 1] - st;
+    }
+    uint32_t ex, ej, d, P, ne, t3;
+    block_excl_sum3<16>(len, len != 0u ? 1u : 0u, 0u, scr + 64 * sp, ex, ej, d, P, ne, t3);
+    sp ^= 1u;
+    if (len) {
+      cex[ej] = ex;
+      cq[ej] = st - ex;
+    }
+    if (tid == 0) cex[ne] = P;
+    __syncthreads();  // segments (and, on the first chunk, the cleared bitmap)
+    const uint32_t per = ((P + 15) / 16 + 63) & ~63u;
+    const uint32_t beg = min(P, wid * per), end = min(P, beg + per);
+    if (beg < end) {
+      uint32_t e_base;
+      {
+        const uint32_t S = (ne + 63) / 64;
+        const uint32_t i1 = lane * S;
+        const uint64_t b1 = __ballot(i1 < ne && cex[min(i1, ne - 1)] <= beg);
+        const uint32_t L1 = 63 - __clzll((long long)b1);
+        const uint32_t i2 = L1 * S + lane;
+        const uint64_t b2 = __ballot(lane < S && i2 < ne && cex[min(i2, ne - 1)] <= beg);
+        e_base = L1 * S + (63 - __clzll((long long)b2));
+      }
+      uint32_t nxw = e_base + 1 + lane < ne ? cex[e_base + 1 + lane] : kEmpty;
+      for (uint32_t pb = beg; pb < end; pb += 64 * U) {
+        uint32_t own[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint32_t p0 = pb + 64 * u;
+          own[u] = 0;
+          if (p0 < end) {
+            const uint32_t p = p0 + lane;
+            uint64_t mlo = __ballot(nxw <= p0);
+            if ((uint32_t)__builtin_amdgcn_readlane((int)nxw, kWave - 1) <= p0 + 63) {
+              e_base += (uint32_t)__popcll(mlo);
+              nxw = e_base + 1 + lane < ne ? cex[e_base + 1 + lane] : kEmpty;
+              mlo = 0;
+            }
+            uint64_t cand = __ballot(nxw <= p0 + 63) & ~mlo;
+            uint32_t ow = e_base + (uint32_t)__popcll(mlo);
+            while (cand) {
+              const int i = __builtin_ctzll(cand);
+              cand &= cand - 1;
+              ow += (uint32_t)__builtin_amdgcn_readlane((int)nxw, i) <= p ? 1u : 0u;
+            }
+            own[u] = ow;
+          }
+        }
+        uint32_t col[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint32_t p = pb + 64 * u + lane;
+          col[u] = p < end ? a.b_ci[cq[own[u]] + p] : kEmpty;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+          if (col[u] != kEmpty) {
+            const uint32_t x = col[u] - lo;
+            atomicOr(&bits[x >> 5], 1u << (x & 31));
+          }
+      }
+    }
+    __syncthreads();  // marks done; the next chunk rewrites the segments
+  }
+  uint32_t cnt = 0;
+  for (uint32_t i = tid; i < n4; i += BS) {
+    const uint4 v = bits4[i];
+    cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+    if (bo != ~0ull) bm[bo + i] = v;
+  }
+  uint32_t ec, d1, d2, tot, t1, t2;
+  block_excl_sum3<16>(cnt, 0u, 0u, scr + 64 * sp, ec, d1, d2, tot, t1, t2);
+  if (tid == 0) nnz[r] = tot;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sym_flat(KArgs<float> a, const uint32_t *__restrict__ rows,
+                                                 const uint32_t *__restrict__ bounds,
+                                                 uint32_t *__restrict__ nnz,
+                                                 const uint64_t *__restrict__ bmoff,
+                                                 uint4 *__restrict__ bm) {
+  if (!bounds) {
+    sym_flat_row<BS>(a, rows[blockIdx.x], nnz, bmoff, bm);
+    return;
+  }
+  // persistent: the rows [bounds[0], bounds[1]) of the list
+  const uint32_t b1 = bounds[1];
+  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
+    sym_flat_row<BS>(a, rows[i], nnz, bmoff, bm);
+    __syncthreads();  // the next row re-initialises the LDS
+  }
+}
+
+uint32_t sym_flat_entry_bytes() { return kFlatEntryBytes; }
+
+int launch_sym_flat(uint32_t span_bits, const AccArgs &a, const uint32_t *rows, uint32_t nrows,
+                    uint32_t *nnz, hipStream_t s, const uint64_t *bmoff, void *bm,
+                    const RowSel &sel) {
+  if (nrows == 0 && !sel.bounds) return OC_OK;
+  const Tables &t = tables();
+  const size_t lds = 512 + kFlatEntryBytes + ((size_t)(span_bits + 126) / 128 + 1) * 16;
+  if (lds > t.lds_max_block) {
+    set_error("flat dense symbolic kernel needs %zu B of LDS, device allows %u", lds,
+              t.lds_max_block);
+    return OC_ERR_INVALID;
+  }
+  OC_TRY(set_lds(k_sym_flat<kR1Block>, lds));
+  const uint32_t grid = sel.bounds ? std::max<uint32_t>(1, t.num_cu) : nrows;
+  hipLaunchKernelGGL(k_sym_flat<kR1Block>, dim3(grid), dim3(kR1Block), lds, s, kargs<float>(a),
+                     rows, sel.bounds, nnz, bmoff, (uint4 *)bm);
+  LAUNCH_CHECK();
+  return OC_OK;
+}
 
 // ------------------------------------------------------ stored row bitmaps
 // Rows of the dense symbolic bins with at least min_products products get

@@ -31,9 +31,21 @@ struct Tables {
   uint32_t win_sym_cols;   // symbolic window (columns)
   uint32_t lds_per_cu, num_cu, lds_max_block;
 };
-// Tables for the current device (nominal 160 KB / 256 CUs without a GPU).
+// Tables for the current device (nominal 160 KB / 256 CUs without a GPU),
+// or the ones a TablesScope on this thread set.
 const Tables &tables();
 int tables_init();
+// Tables for per-bin shapes (workgroups per CU, threads per workgroup;
+// null or 0 entries: the defaults); an error when a bin's tables do not fit.
+int tables_for(const int32_t *concurrency, const int32_t *block, Tables *out);
+// While it lives, tables() on this thread returns *t (t null: no change).
+struct TablesScope {
+  const Tables *prev;
+  explicit TablesScope(const Tables *t);
+  ~TablesScope();
+  TablesScope(const TablesScope &) = delete;
+  TablesScope &operator=(const TablesScope &) = delete;
+};
 
 // Per-row analysis outputs.
 struct RowInfo {
@@ -53,6 +65,11 @@ size_t analysis_temp_bytes(uint64_t a_rows, uint64_t a_nnz, uint64_t b_rows);
 int launch_analysis(const DevCsr &A, const DevCsr &B, const RowInfo &ri,
                     uint32_t *max_a_nnz, uint32_t *max_products, void *temp,
                     hipStream_t s);
+// Device-side checks: *err |= bit when the u32 (is64 0) or u64 at x
+// exceeds limit; and a fill of n u32 (a kernel, not a memset node).
+int launch_check_gt(const void *x, int is64, uint64_t limit, uint32_t bit, uint32_t *err,
+                    hipStream_t s);
+int launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 size_t scan_temp_bytes(uint64_t n);
 int launch_scan_u32(const uint32_t *in, uint64_t *out, uint64_t n, void *temp,
                     hipStream_t s);
@@ -132,30 +149,45 @@ struct NumOut {
   const uint32_t *nnz_hint;  // exact per-row counts when known (or null)
 };
 
+// Rows of a launch taken from the device instead of the host: with bounds
+// set, the kernel is persistent (as many workgroups as fit the chip) over
+// rows[bounds[0] .. bounds[1]) of its list, so the launch needs no host
+// read-back of the bin sizes (a whole run can be captured into a graph).
+// err (numeric kernels): when *err != 0 at launch, nothing is written (the
+// output's capacity check failed).
+struct RowSel {
+  const uint32_t *bounds = nullptr;
+  const uint32_t *err = nullptr;
+};
+
 // Symbolic: rows of one bin.  Overflowing rows (estimated-compaction hash
 // tables that fill up) are appended to ovf_rows.
 int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
                     uint32_t nrows, uint32_t limit, uint32_t *nnz,
-                    uint32_t *ovf_rows, uint32_t *ovf_count, hipStream_t s);
+                    uint32_t *ovf_rows, uint32_t *ovf_count, hipStream_t s,
+                    const RowSel &sel = RowSel{});
 // bmoff/bm (optional): rows with bmoff[r] != ~0 store their bitmap (uint4
 // groups from column left & ~127) at bm + bmoff[r] (launch_bitmap_offsets)
 int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
                      uint32_t nrows, uint32_t *nnz, hipStream_t s,
-                     const uint64_t *bmoff = nullptr, void *bm = nullptr);
+                     const uint64_t *bmoff = nullptr, void *bm = nullptr,
+                     const RowSel &sel = RowSel{});
 // Windowed kernels take the bin row list (device) and its host-known
 // length; cursor scratch is grid * max_a_nnz u32.
 int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                       uint32_t max_a_nnz, uint32_t *nnz, void *scratch,
-                      hipStream_t s);
+                      hipStream_t s, const RowSel &sel = RowSel{});
 size_t window_scratch_bytes(uint32_t max_a_nnz);
 
 int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
-                    uint32_t nrows, const NumOut &o, hipStream_t s);
+                    uint32_t nrows, const NumOut &o, hipStream_t s,
+                    const RowSel &sel = RowSel{});
 int launch_num_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
-                     uint32_t nrows, const NumOut &o, hipStream_t s);
+                     uint32_t nrows, const NumOut &o, hipStream_t s,
+                     const RowSel &sel = RowSel{});
 int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                       uint32_t max_a_nnz, const NumOut &o, void *scratch,
-                      hipStream_t s);
+                      hipStream_t s, const RowSel &sel = RowSel{});
 // Tiled numeric path for rows too large for LDS hash / dense kernels:
 // tile_plan picks the tile width (2^logW columns) and the size of the
 // tile-start table (B.rows x T1 u32); returns 0 when tiles do not apply.
@@ -176,16 +208,25 @@ int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row);
 // block-diagonal product; the tile tables are then built from the same bases.
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
                     const uint32_t *ts, uint32_t T1, const uint32_t *tsw, const uint32_t *bbase,
-                    uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s);
-// One-gather rank tiles (k_rank.hip), 1024 threads, tiles up to
-// 2^rank1_wide_log() columns; tsw only when built at that width.  Rows
+                    uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s,
+                    const RowSel &sel = RowSel{});
+// Flat dense symbolic kernel (k_rank.hip) for the widest dense symbolic bin
+// (1024 threads): spans up to span_bits, LDS = 512 + sym_flat_entry_bytes()
+// + the bitmap; bitmap store as launch_sym_dense.
+uint32_t sym_flat_entry_bytes();
+int launch_sym_flat(uint32_t span_bits, const AccArgs &a, const uint32_t *rows, uint32_t nrows,
+                    uint32_t *nnz, hipStream_t s, const uint64_t *bmoff, void *bm,
+                    const RowSel &sel);
+// One-gather rank tiles (k_rank.hip), 1024 or 512 threads (bs), tiles up
+// to 2^rank1_wide_log(bs) columns; tsw only when built at that width.  Rows
 // with a stored bitmap (bmoff[r] != ~0, no block bases) gather each
 // product once; the others mark their tiles first.
-uint32_t rank1_wide_log();
+uint32_t rank1_wide_log(uint32_t bs);
 int launch_num_rank1(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
                      const uint32_t *ts, uint32_t T1, const uint32_t *tsw,
                      const uint32_t *bbase, const void *bm, const uint64_t *bmoff,
-                     const NumOut &o, void *scratch, hipStream_t s);
+                     const NumOut &o, void *scratch, hipStream_t s,
+                     const RowSel &sel = RowSel{}, uint32_t bs = 1024);
 // Stored-bitmap offsets (uint4 units) for the listed rows with at least
 // min_products products; bmoff[0..m) = ~0 elsewhere; offs[n] = total.
 // sz: n u32, offs: n + 1 u64, scan_tmp: scan_temp_bytes(n).

@@ -236,6 +236,18 @@ struct oc_spgemm {
     const uint32_t *block_base = nullptr;
     void *row_bm = nullptr;
     const uint64_t *row_bmoff = nullptr;
+    // what the whole-path replay (analysis to numeric, on the device) needs
+    uint32_t *scal = nullptr;           // analysis maxima
+    void *ana_tmp = nullptr, *scan_tmp = nullptr, *bin_tmp = nullptr;
+    uint8_t *bor1 = nullptr, *bor2 = nullptr;
+    uint32_t *brows1 = nullptr, *boff1 = nullptr, *boff2 = nullptr;
+    uint32_t *nnz = nullptr, *ovf = nullptr;
+    uint32_t *err = nullptr;            // check bits of the last whole-path run
+    uint64_t cap_nnz = 0;               // C's capacity (the planned nnz)
+    uint32_t tsw_log = 0, tsw_per = 0;  // compact tile-start table shape
+    BinParams sym_bp{}, num_bp{};
+    hipGraph_t graph_full = nullptr;
+    hipGraphExec_t exec_full = nullptr;
     uint32_t tile_logw = 0, tile_t1 = 0, max_a_nnz = 0;
     int tiles_ok = -1;
     void *win_scratch = nullptr;
@@ -246,10 +258,15 @@ struct oc_spgemm {
     hipGraphExec_t exec = nullptr;
   } plan;
   bool keep_plan = false;
+  // per-bin shapes from the config (bin_concurrency / bin_block)
+  Tables shapes{};
+  bool custom_shapes = false;
   hipStream_t own_stream = nullptr;  // capture needs a non-default stream
   void drop_plan() {
     if (plan.exec) (void)hipGraphExecDestroy(plan.exec);
     if (plan.graph) (void)hipGraphDestroy(plan.graph);
+    if (plan.exec_full) (void)hipGraphExecDestroy(plan.exec_full);
+    if (plan.graph_full) (void)hipGraphDestroy(plan.graph_full);
     if (!plan.mem.empty()) (void)hipDeviceSynchronize();
     for (void *p : plan.mem) (void)hipFree(p);
     plan = Plan();
@@ -354,6 +371,7 @@ void decide_sampling(const oc_config &c, const oc_spgemm &h,
 // ------------------------------------------------------------- run state
 struct Run {
   oc_spgemm *h;
+  TablesScope tscope;  // the handle's bin shapes for every launch of this run
   const oc_config &c;
   hipStream_t s;
   Arena ar;
@@ -373,6 +391,10 @@ struct Run {
   // row bitmaps stored by the dense symbolic pass (one-gather rank tiles)
   void *row_bm = nullptr;
   const uint64_t *row_bmoff = nullptr;
+  void *ana_tmp = nullptr;  // analysis temporaries (kept for a plan)
+  void *bin_tmp = nullptr;  // temporaries of the last binning
+  uint32_t tsw_log = 0, tsw_per = 0;
+  RowSel sel_big{};         // device row range of the long rows (whole-path replay)
   // long rows in column bands (precise workflow)
   struct Band {
     uint32_t n = 0, nbands = 0, logw = 0;
@@ -393,7 +415,8 @@ struct Run {
     if (c.track_stage_times && n && !replay) fams.push_back(Fam{name, stage, numeric, rows, n});
   }
   Run(oc_spgemm *hh, hipStream_t st, oc_csr *cc)
-      : h(hh), c(hh->cfg), s(st), ar(&hh->mem), mk(st), C(cc) {}
+      : h(hh), tscope(hh->custom_shapes ? &hh->shapes : nullptr), c(hh->cfg), s(st),
+        ar(&hh->mem), mk(st), C(cc) {}
 
   AccArgs acc() const {
     AccArgs a;
@@ -435,12 +458,14 @@ struct Run {
       OC_TRY(ar.get(&tile_ts, bytes / 4));
       OC_TRY(launch_tile_starts(B, tile_logw, tile_t1, tile_ts, s, block_base));
       size_t per = 0;
-      // the one-gather kernel always runs 1024-thread workgroups
-      const uint32_t bs = mode == 3 ? 1024u : (uint32_t)h->cfg.tile_block;
+      // the one-gather kernel runs 1024- or 512-thread workgroups
+      const uint32_t bs = mode == 3 ? (h->cfg.tile_block == 512 ? 512u : 1024u)
+                                    : (uint32_t)h->cfg.tile_block;
       if (mode >= 2 && wide_starts_plan(width, bs, &per)) {
         OC_TRY(ar.get(&tile_tsw, B.rows * per));
-        OC_TRY(launch_tile_starts(B, rank_wide_log(bs), (uint32_t)per, tile_tsw, s,
-                                  block_base));
+        tsw_log = rank_wide_log(bs);
+        tsw_per = (uint32_t)per;
+        OC_TRY(launch_tile_starts(B, tsw_log, tsw_per, tile_tsw, s, block_base));
       }
       mk.mark("numeric_tile_starts");
     }
@@ -477,23 +502,32 @@ struct Run {
     return OC_OK;
   }
 
-  int big_rows(const uint32_t *rows, uint32_t n, const NumOut &o) {
-    if (!n) return OC_OK;
+  // sel: the device row range of a whole-path replay (n is then unused)
+  int big_rows(const uint32_t *rows, uint32_t n, const NumOut &o, const RowSel &sel = RowSel{}) {
+    if (!n && !sel.bounds) return OC_OK;
     OC_TRY(window_scratch());
     OC_TRY(tiles());
-    if (tiles_ok && h->cfg.use_tiled_numeric == 3)
+    if (tiles_ok && h->cfg.use_tiled_numeric == 3) {
+      const uint32_t bs = h->cfg.tile_block == 512 ? 512u : 1024u;
       return launch_num_rank1(acc(), rows, n, tile_logw, tile_ts, tile_t1,
-                              rank_wide_log(1024) == rank1_wide_log() ? tile_tsw : nullptr,
-                              block_base, row_bm, row_bmoff, o, win_scratch, s);
+                              tsw_log == rank1_wide_log(bs) ? tile_tsw : nullptr,
+                              block_base, row_bm, row_bmoff, o, win_scratch, s, sel, bs);
+    }
     if (tiles_ok && h->cfg.use_tiled_numeric == 2)
       return launch_num_rank(acc(), rows, n, tile_logw, tile_ts, tile_t1, tile_tsw,
                              block_base, (uint32_t)h->cfg.tile_products_cap,
                              (uint32_t)h->cfg.tile_block, o,
-                             win_scratch, s);
-    if (tiles_ok)
+                             win_scratch, s, sel);
+    if (tiles_ok) {
+      if (sel.bounds) {
+        set_error("the dense-tile long-row kernel (use_tiled_numeric 1) has no "
+                  "whole-path replay");
+        return OC_ERR_INVALID;
+      }
       return launch_num_tiles(acc(), rows, n, tile_logw, (uint32_t)h->cfg.tile_block, tile_ts,
                               tile_t1, o, win_scratch, s);
-    return launch_num_window(acc(), rows, n, h->max_a_nnz, o, win_scratch, s);
+    }
+    return launch_num_window(acc(), rows, n, h->max_a_nnz, o, win_scratch, s, sel);
   }
 
   // C: u64 offsets (rows+1) from per-row counts, then cols/values.
@@ -528,6 +562,7 @@ struct Run {
     }
     uint8_t *tmp;
     OC_TRY(ar.get(&tmp, bin_temp_bytes(m)));
+    bin_tmp = tmp;
     OC_TRY(launch_binning(bp, m, *bor, *brows, *boff, tmp, s));
     OC_TRY(readback(hoff, *boff, kBinSlots + 1, s));
     return OC_OK;
@@ -657,7 +692,10 @@ int Run::prologue_and_analysis() {
   uint8_t *atmp;
   OC_TRY(ar.get(&atmp, analysis_temp_bytes(m, A.nnz, B.rows)));
   OC_TRY(launch_analysis(A, B, ri, scal, scal + 1, atmp, s));
-  ar.free(atmp);  // stream-ordered: released once the analysis has run
+  ana_tmp = atmp;
+  // stream-ordered: released once the analysis has run, unless a plan
+  // keeps it for whole-path replays
+  if (!h->keep_plan) ar.free(atmp);
   mk.mark("analysis_product_calc");
   OC_TRY(launch_scan_u32(ri.products, psum_prod, m, scan_tmp, s));
   mk.mark("analysis_reduce");
@@ -1072,6 +1110,10 @@ int Run::run_precise() {
   h->precise_recount = n;
   OC_TRY(big_rows(ovf_rows, n, o));
   if (h->keep_plan) {
+    // a whole-path replay needs the window scratch and the tile tables
+    // even when this run had no rows for them
+    OC_TRY(window_scratch());
+    OC_TRY(tiles());
     oc_spgemm::Plan &p = h->plan;
     p.A = A;
     p.B = B;
@@ -1084,6 +1126,24 @@ int Run::run_precise() {
     p.block_base = block_base;
     p.row_bm = row_bm;
     p.row_bmoff = row_bmoff;
+    p.tsw_log = tsw_log;
+    p.tsw_per = tsw_per;
+    p.scal = scal;
+    p.ana_tmp = ana_tmp;
+    p.scan_tmp = scan_tmp;
+    p.bin_tmp = bin_tmp;
+    p.bor1 = bor;
+    p.brows1 = brows;
+    p.boff1 = boff;
+    p.bor2 = bor2;
+    p.boff2 = boff2;
+    p.nnz = nnz;
+    p.ovf = ovf;
+    p.cap_nnz = C->nnz;
+    p.sym_bp = bp;
+    p.sym_bp.use_compaction = 0;  // whole-path replays use full tables (no overflow)
+    p.num_bp = nb;
+    if (!p.err) OC_TRY(ar.get(&p.err, 64));
     p.tile_logw = tile_logw;
     p.tile_t1 = tile_t1;
     p.tiles_ok = tiles_ok;
@@ -1694,6 +1754,8 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   r.block_base = p.block_base;
   r.row_bm = p.row_bm;
   r.row_bmoff = p.row_bmoff;
+  r.tsw_log = p.tsw_log;
+  r.tsw_per = p.tsw_per;
   r.tile_logw = p.tile_logw;
   r.tile_t1 = p.tile_t1;
   r.tiles_ok = p.tiles_ok;
@@ -1717,12 +1779,107 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   return rc;
 }
 
+// Check bits of a whole-path replay (oc_spgemm_full_status).
+constexpr uint32_t kFullCapacity = 1;   // nnz(C) beyond the planned capacity
+constexpr uint32_t kFullLongA = 2;      // an A row longer than the planned longest
+constexpr uint32_t kFullSymOvf = 4;     // a symbolic table overflowed
+constexpr uint32_t kFullNumOvf = 8;     // a numeric row needed a recount
+
+// The whole precise path of a kept plan on stream s, from the analysis to
+// the numeric kernels, for the current contents of the planned A and B
+// (same dimensions and entry counts; structure and values may change).  Bin
+// sizes stay on the device: every bin kernel runs persistent over its
+// device-side row range, so nothing is read back, allocated or set with a
+// memset node and the sequence can be captured into one graph.  C keeps the
+// planned capacity; when a product needs more (or a row outgrows a planned
+// buffer) the check bits are set and the numeric kernels write nothing.
+int full_replay_impl(oc_spgemm *h, hipStream_t s) {
+  oc_spgemm::Plan &p = h->plan;
+  OC_REQUIRE(p.valid, "no numeric plan: run oc_spgemm_plan first");
+  OC_REQUIRE(p.band_n == 0 && h->cfg.band_log2_cols == 0,
+             "the whole-path replay does not support band-major long rows");
+  OC_REQUIRE(p.ana_tmp && p.bin_tmp && p.err && p.boff1 && p.boff2,
+             "internal: the plan holds no whole-path state");
+  Run r(h, s, nullptr);
+  r.replay = true;
+  r.mk.enabled = false;
+  r.A = p.A;
+  r.B = p.B;
+  r.ri = p.ri;
+  r.scal = p.scal;
+  r.scan_tmp = p.scan_tmp;
+  r.tile_ts = p.tile_ts;
+  r.tile_tsw = p.tile_tsw;
+  r.block_base = p.block_base;
+  r.row_bm = nullptr;  // the stored bitmaps belong to the planned structure
+  r.row_bmoff = nullptr;
+  r.tsw_log = p.tsw_log;
+  r.tsw_per = p.tsw_per;
+  r.tile_logw = p.tile_logw;
+  r.tile_t1 = p.tile_t1;
+  r.tiles_ok = p.tiles_ok;
+  r.win_scratch = p.win_scratch;
+  const uint64_t m = p.A.rows;
+  uint32_t *err = p.err;
+  const AccArgs a = r.acc();
+  const Tables &t = tables();
+  OC_TRY(launch_fill_u32(err, 1, 0u, s));
+  OC_TRY(launch_fill_u32(p.ovf, 8, 0u, s));
+  // analysis (fixed grid) and the longest A row against the planned one
+  OC_TRY(launch_analysis(p.A, p.B, p.ri, p.scal, p.scal + 1, p.ana_tmp, s));
+  OC_TRY(launch_check_gt(p.scal, 0, p.max_a_nnz, kFullLongA, err, s));
+  // tile-start tables of the long-row kernels follow B's structure
+  if (p.tiles_ok > 0) {
+    OC_TRY(launch_tile_starts(p.B, p.tile_logw, p.tile_t1, p.tile_ts, s, p.block_base));
+    if (p.tile_tsw)
+      OC_TRY(launch_tile_starts(p.B, p.tsw_log, p.tsw_per, p.tile_tsw, s, p.block_base));
+  }
+  // symbolic: binning, then every bin persistent over its device range
+  OC_TRY(launch_binning(p.sym_bp, m, p.bor1, p.brows1, p.boff1, p.bin_tmp, s));
+  uint32_t *ovf_rows = p.ovf + 8;
+  for (int b = 0; b <= 5; ++b)
+    OC_TRY(launch_sym_hash(b, a, p.brows1, 0, t.hash_sym[b], p.nnz, ovf_rows, p.ovf, s,
+                           RowSel{p.boff1 + b, nullptr}));
+  OC_TRY(launch_sym_window(a, p.brows1, 0, p.max_a_nnz, p.nnz, p.win_scratch, s,
+                           RowSel{p.boff1 + 6, err}));
+  for (int b = 8; b <= 13; ++b)
+    OC_TRY(launch_sym_dense(b - 7, a, p.brows1, 0, p.nnz, s, nullptr, nullptr,
+                            RowSel{p.boff1 + b, nullptr}));
+  OC_TRY(launch_check_gt(p.ovf, 0, 0, kFullSymOvf, err, s));
+  // C's offsets, and its capacity against the new nnz
+  OC_TRY(launch_scan_u32(p.nnz, (uint64_t *)p.o.off, m, p.scan_tmp, s));
+  OC_TRY(launch_check_gt(p.o.off + m, 1, p.cap_nnz, kFullCapacity, err, s));
+  // numeric: binning on the exact counts, then every bin persistent
+  OC_TRY(launch_binning(p.num_bp, m, p.bor2, const_cast<uint32_t *>(p.brows), p.boff2, p.bin_tmp, s));
+  for (int b = 1; b <= 6; ++b)
+    OC_TRY(launch_num_hash(b, a, p.brows, 0, p.o, s, RowSel{p.boff2 + b, err}));
+  for (int b = 7; b <= 12; ++b)
+    OC_TRY(launch_num_dense(b - 7, a, p.brows, 0, p.o, s, RowSel{p.boff2 + b, err}));
+  const uint32_t keep_max = h->max_a_nnz;
+  h->max_a_nnz = p.max_a_nnz;
+  int rc = r.big_rows(p.brows, 0, p.o, RowSel{p.boff2 + 13, err});
+  h->max_a_nnz = keep_max;
+  OC_TRY(rc);
+  OC_TRY(launch_check_gt(p.ovf + 1, 0, 0, kFullNumOvf, err, s));
+  if (!r.ar.recs.empty()) {
+    set_error("internal: a whole-path replay allocated memory");
+    return OC_ERR_INVALID;
+  }
+  return OC_OK;
+}
+
 }  // namespace
 
 namespace oc {
 // Used by the batched layer (batch.hip) to record the numeric phase into
 // its own graph after refreshing the concatenated values.
 int spgemm_replay_async(oc_spgemm *h, hipStream_t s) { return replay_impl(h, s); }
+int spgemm_full_replay_async(oc_spgemm *h, hipStream_t s) { return full_replay_impl(h, s); }
+int spgemm_full_status(oc_spgemm *h, uint32_t *status) {
+  OC_REQUIRE(h && h->plan.valid && h->plan.err, "no whole-path plan");
+  HIP_TRY(hipMemcpy(status, h->plan.err, 4, hipMemcpyDeviceToHost));
+  return OC_OK;
+}
 void spgemm_set_blocks(oc_spgemm *h, const uint32_t *b_block_base, uint64_t span,
                        const void *b_row_offsets) {
   if (!h) return;
@@ -1747,6 +1904,13 @@ extern "C" oc_spgemm *oc_spgemm_create(const oc_config *cfg) {
   else oc_config_default(&h->cfg);
   if (oc_config_check(&h->cfg) != OC_OK) {
     delete h;  // the check's message stays set
+    return nullptr;
+  }
+  for (int b = 0; b < 7; ++b)
+    if (h->cfg.bin_concurrency[b] || h->cfg.bin_block[b]) h->custom_shapes = true;
+  if (h->custom_shapes &&
+      tables_for(h->cfg.bin_concurrency, h->cfg.bin_block, &h->shapes) != OC_OK) {
+    delete h;  // the message names the bin
     return nullptr;
   }
   if (h->cfg.use_learned_binning && h->cfg.learned_model_file[0]) {
@@ -1842,6 +2006,55 @@ extern "C" int oc_spgemm_capture(oc_spgemm *h) {
   p.graph = g;
   HIP_TRY(hipGraphInstantiate(&p.exec, g, nullptr, nullptr, 0));
   return OC_OK;
+}
+
+extern "C" int oc_spgemm_capture_full(oc_spgemm *h) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  OC_REQUIRE(h->plan.valid, "no numeric plan: run oc_spgemm_plan first");
+  oc_spgemm::Plan &p = h->plan;
+  if (p.exec_full) (void)hipGraphExecDestroy(p.exec_full);
+  if (p.graph_full) (void)hipGraphDestroy(p.graph_full);
+  p.exec_full = nullptr;
+  p.graph_full = nullptr;
+  if (!h->own_stream) HIP_TRY(hipStreamCreate(&h->own_stream));
+  HIP_TRY(hipStreamBeginCapture(h->own_stream, hipStreamCaptureModeRelaxed));
+  const int rc = full_replay_impl(h, h->own_stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->own_stream, &g);
+  if (rc != OC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_TRY(e);
+  p.graph_full = g;
+  HIP_TRY(hipGraphInstantiate(&p.exec_full, g, nullptr, nullptr, 0));
+  return OC_OK;
+}
+
+extern "C" int oc_spgemm_graph_launch_full(oc_spgemm *h, void *stream) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  OC_REQUIRE(h->plan.valid && h->plan.exec_full,
+             "no whole-path graph: run oc_spgemm_capture_full first");
+  if (stream) {
+    HIP_TRY(hipGraphLaunch(h->plan.exec_full, (hipStream_t)stream));
+  } else {
+    HIP_TRY(hipGraphLaunch(h->plan.exec_full, h->own_stream));
+    HIP_TRY(hipStreamSynchronize(h->own_stream));
+  }
+  return OC_OK;
+}
+
+extern "C" int oc_spgemm_full_replay(oc_spgemm *h, void *stream) {
+  OC_REQUIRE(h != nullptr, "null handle");
+  hipStream_t s = (hipStream_t)stream;
+  OC_TRY(full_replay_impl(h, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return OC_OK;
+}
+
+extern "C" int oc_spgemm_full_status(oc_spgemm *h, uint32_t *status) {
+  OC_REQUIRE(status != nullptr, "null status");
+  return spgemm_full_status(h, status);
 }
 
 extern "C" int oc_spgemm_graph_launch(oc_spgemm *h, void *stream) {

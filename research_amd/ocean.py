@@ -71,6 +71,7 @@ class _Config(C.Structure):
         ("learned_quantile", C.c_double),
         ("band_log2_cols", C.c_int32), ("band_min_products", C.c_int32),
         ("learned_min_compression", C.c_double),
+        ("bin_concurrency", C.c_int32 * 7), ("bin_block", C.c_int32 * 7),
     ]
 
 
@@ -172,6 +173,13 @@ def _load():
         "oc_batch_launch": (I, [P, P]),
         "oc_batch_result": (I, [P, P, P, P]),
         "oc_batch_extract": (I, [P, U32, P, P]),
+        "oc_spgemm_full_replay": (I, [P, P]),
+        "oc_spgemm_capture_full": (I, [P]),
+        "oc_spgemm_graph_launch_full": (I, [P, P]),
+        "oc_spgemm_full_status": (I, [P, P]),
+        "oc_batch_capture_full": (I, [P]),
+        "oc_batch_launch_full": (I, [P, P]),
+        "oc_batch_full_status": (I, [P, P]),
         "oc_batch_handle": (P, [P]),
     }
     for name, (res, args) in sig.items():
@@ -221,6 +229,8 @@ class Config:
         if k.startswith("_"):
             raise AttributeError(k)
         v = getattr(self._c, k)
+        if isinstance(v, C.Array) and not isinstance(v, bytes):
+            return list(v)
         return v.decode() if isinstance(v, bytes) else v
 
     def __setattr__(self, k, v):
@@ -229,6 +239,13 @@ class Config:
             return
         if not hasattr(self._c, k):
             raise AttributeError(f"unknown config field {k}")
+        cur = getattr(self._c, k)
+        if isinstance(cur, C.Array) and isinstance(v, (list, tuple)):
+            if len(v) != len(cur):
+                raise ValueError(f"config field {k} takes {len(cur)} values")
+            for i, x in enumerate(v):
+                cur[i] = x
+            return
         setattr(self._c, k, v.encode() if isinstance(v, str) else v)
 
     def as_dict(self) -> dict:
@@ -474,6 +491,21 @@ class SpGEMM:
     def graph_launch(self, stream=None):
         _check(lib.oc_spgemm_graph_launch(self.h, stream))
 
+    def full_replay(self, stream=None):
+        """The whole path of the planned product on the current A and B."""
+        _check(lib.oc_spgemm_full_replay(self.h, stream))
+
+    def capture_full(self):
+        _check(lib.oc_spgemm_capture_full(self.h))
+
+    def graph_launch_full(self, stream=None):
+        _check(lib.oc_spgemm_graph_launch_full(self.h, stream))
+
+    def full_status(self) -> int:
+        st = C.c_uint32(0)
+        _check(lib.oc_spgemm_full_status(self.h, C.byref(st)))
+        return st.value
+
     def set_model(self, model: "Mlp | None", quantile: float = 0.99):
         """Learned capacities in place of HyperLogLog (None: back to HLL)."""
         self._model = model
@@ -532,6 +564,17 @@ class Batch:
 
     def launch(self, stream=None):
         _check(lib.oc_batch_launch(self.h, stream))
+
+    def capture_full(self):
+        _check(lib.oc_batch_capture_full(self.h))
+
+    def launch_full(self, stream=None):
+        _check(lib.oc_batch_launch_full(self.h, stream))
+
+    def full_status(self) -> int:
+        st = C.c_uint32(0)
+        _check(lib.oc_batch_full_status(self.h, C.byref(st)))
+        return st.value
 
     def result(self):
         """(view of C, row_base[count+1], col_base[count+1])."""

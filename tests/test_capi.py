@@ -37,7 +37,8 @@ def test_tables_gfx950():
     # 160 KB of LDS per CU: per-bin LDS budgets and the table sizes they give
     assert t["lds_bytes"] == [5120, 5120, 10240, 20480, 40960, 81920, 163840]
     assert t["hash_sym"][5] == (81920 - 256) // 4 // 8 * 8
-    assert t["dense_sym"][6] == (163840 - 512) * 8 and t["dense_sym"][0] == 0
+    # the widest dense symbolic bin leaves 9216 B for the flat kernel's segments
+    assert t["dense_sym"][6] == (163840 - 512 - 9216) * 8 and t["dense_sym"][0] == 0
     assert t["hash_num"][0] == 0 and t["dense_num"][6] == 0xFFFFFFFF
     assert all(t["hash_num"][b] < t["hash_num"][b + 1] for b in range(1, 6))
 
@@ -62,3 +63,28 @@ def test_config_range_checks():
         assert ocean.lib.oc_config_from_json(ctypes.byref(cfg()), text) == ocean.OC_ERR_INVALID
     assert not ocean.lib.oc_spgemm_create(ctypes.byref(cfg(symbolic_expansion_coe=0.75)))
     assert b"symbolic_expansion_coe" in ocean.lib.oc_last_error()
+
+
+def test_bin_shapes_config():
+    """Per-bin shapes: JSON arrays of 7 integers, range checks, and tables
+    that must fit each bin's LDS (checked when the handle is created)."""
+    c = ocean.Config.from_json('{"bin_concurrency": [0, 0, 8, 0, 0, 0, 0], '
+                               '"bin_block": [0, 128, 0, 0, 0, 1024, 0], "tile_block": 512}')
+    assert c.bin_concurrency == [0, 0, 8, 0, 0, 0, 0]
+    assert c.bin_block == [0, 128, 0, 0, 0, 1024, 0] and c.tile_block == 512
+    base = ocean._Config()
+    ocean.lib.oc_config_default(ctypes.byref(base))
+    for text in (b'{"bin_block": [0, 0, 0, 300, 0, 0, 0]}', b'{"bin_concurrency": [65, 0, 0, 0, 0, 0, 0]}',
+                 b'{"bin_block": [0, 0, 0]}'):
+        assert ocean.lib.oc_config_from_json(ctypes.byref(base), text) == ocean.OC_ERR_INVALID, text
+    # nested structures other than flat number arrays are still skipped
+    fresh = ocean._Config()
+    ocean.lib.oc_config_default(ctypes.byref(fresh))
+    assert ocean.lib.oc_config_from_json(ctypes.byref(fresh), b'{"x": [1, [2]], "y": {"z": 1}}') == \
+        ocean.OC_OK
+    # 64 workgroups per CU leave 2.5 KB each: too little for a 1024-thread
+    # numeric hash layout
+    bad = ocean.Config(bin_concurrency=[0, 0, 0, 0, 0, 0, 64], bin_block=[0, 0, 0, 0, 0, 0, 1024])
+    assert ocean.lib.oc_config_check(ctypes.byref(bad._c)) == ocean.OC_OK
+    assert not ocean.lib.oc_spgemm_create(ctypes.byref(bad._c))
+    assert b"bin 6" in ocean.lib.oc_last_error()

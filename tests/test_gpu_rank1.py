@@ -80,14 +80,17 @@ def short_segments(gpu, seed=5):
     return A, B
 
 
+@pytest.mark.parametrize("block", [1024, 512])
 @pytest.mark.parametrize("workflow", [0, 1])
 @pytest.mark.parametrize("log_t", [14, 11])
-def test_rank1_rmat15(gpu, workflow, log_t):
+def test_rank1_rmat15(gpu, workflow, log_t, block):
     """R-MAT 2^15, degree 32: precise (stored bitmaps) and estimation
-    (tiles marked first, capacities checked) workflows."""
-    log(f"case: rank1 rmat 2^15 deg 32, workflow {workflow}, tiles 2^{log_t}")
+    (tiles marked first, capacities checked) workflows, 1024-thread
+    workgroups (one per CU) and 512-thread ones (two per CU)."""
+    log(f"case: rank1 rmat 2^15 deg 32, workflow {workflow}, tiles 2^{log_t}, block {block}")
     A = gpu.CSR.rmat(15, 32 << 15, seed=5)
-    cfg = gpu.Config(use_tiled_numeric=3, tile_log2_cols=log_t, track_stage_times=1)
+    cfg = gpu.Config(use_tiled_numeric=3, tile_log2_cols=log_t, tile_block=block,
+                     track_stage_times=1)
     C, st = _run(gpu, A, A, cfg, workflow)
     assert st["binning_2"][14] - st["binning_2"][13] > 0, "no rows reached the big-row bin"
     if workflow == 0:
@@ -100,12 +103,13 @@ def test_rank1_rmat15(gpu, workflow, log_t):
 @pytest.mark.parametrize("workflow", [0, 1])
 @pytest.mark.parametrize("vtype", [0, 1])
 def test_rank1_hubs(gpu, workflow, vtype):
-    """Rows of more than 1024 A entries (chunked) and tiles with more
-    outputs than slots (rank ranges), f64 and f32."""
+    """Rows of more than 1024 (or 512) A entries (chunked) and tiles with
+    more outputs than slots (rank ranges), f64 and f32, both workgroup
+    sizes."""
     log(f"case: rank1 hubs, workflow {workflow}, vtype {vtype}")
     A, B = hubs(gpu, vtype=vtype)
-    for log_t in (10, 14):
-        cfg = gpu.Config(use_tiled_numeric=3, tile_log2_cols=log_t)
+    for log_t, block in ((10, 1024), (14, 1024), (12, 512)):
+        cfg = gpu.Config(use_tiled_numeric=3, tile_log2_cols=log_t, tile_block=block)
         C, st = _run(gpu, A, B, cfg, workflow)
         worst, _ = check_product(A, B, C, TOL64 if vtype == 0 else TOL32, f"hubs 2^{log_t}")
         log(f"rank1 hubs 2^{log_t}: nnz {C.nnz}, err {worst:.2e}")

@@ -125,11 +125,12 @@ def test_precise_dense_and_hash_bins(gpu):
     log(f"bins: {st.get('binning_1')} / {st.get('binning_2')}")
 
 
-@pytest.mark.parametrize("tiled,block", [(2, 1024), (2, 512), (2, 256), (1, 1024), (0, 1024)])
+@pytest.mark.parametrize("tiled,block", [(3, 1024), (2, 1024), (2, 512), (2, 256), (1, 1024),
+                                         (0, 1024)])
 def test_big_rows_tiled_and_windowed(gpu, tiled, block):
     """Rows beyond the LDS hash/dense kernels (numeric bin 13) through the
-    rank-tile kernel (three workgroup shapes), the dense-tile kernel and
-    the cursor-windowed kernel."""
+    one-gather rank tiles, the rank-tile kernel (three workgroup shapes),
+    the dense-tile kernel and the cursor-windowed kernel."""
     log(f"case: rmat 2^15 deg 32 f64, big rows, use_tiled_numeric={tiled} block={block}")
     A = gpu.CSR.rmat(15, 32 << 15, seed=5)
     cfg = gpu.Config(use_tiled_numeric=tiled, tile_block=block, band_log2_cols=0,
@@ -156,3 +157,17 @@ def test_band_long_rows(gpu, log_n, deg, log_w, min_products, vtype):
     assert skipped > 0 and "band_numeric" in st["kernels"], "no rows took the band path"
     worst, plain = check_product(A, A, C, TOL64 if vtype == 0 else TOL32, f"bands 2^{log_w}")
     log(f"bands 2^{log_w}: {skipped} long rows, err {worst:.2e}")
+
+
+@pytest.mark.parametrize("conc,block", [([16, 16, 8, 4, 2, 1, 1], [128, 128, 256, 512, 1024, 1024, 1024]),
+                                        ([32, 32, 32, 16, 8, 4, 2], [64, 64, 64, 128, 256, 256, 512])])
+def test_bin_shapes(gpu, conc, block):
+    """Non-default per-bin shapes (twice or half the LDS per workgroup,
+    other workgroup sizes): the tables, bins and kernel instances follow,
+    and the product still equals the oracle's, in both workflows."""
+    log(f"case: bin shapes conc {conc} block {block}")
+    A = gpu.CSR.rmat(14, 24 << 14, seed=9)
+    for wf in (0, 1):
+        C, st = _run(gpu, A, A, gpu.Config(bin_concurrency=conc, bin_block=block), wf)
+        worst, _ = check_product(A, A, C, TOL64, f"shapes wf {wf}")
+        log(f"shapes {block} wf {wf}: bins {st['binning_2'][:14]}, err {worst:.2e}")

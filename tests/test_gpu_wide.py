@@ -2,7 +2,7 @@
 path, and the workflow decisions, against the oracle.
 
 Rows whose column span exceeds the widest dense symbolic bitmap
-(1,306,624 columns) and whose products exceed the hash bins take the
+(tables()["dense_sym"][6] columns) and whose products exceed the hash bins take the
 windowed symbolic kernel (the role of hashSymbolicGMEMKernel,
 kernels/AccumulatorHash.cuh:297-433, launched at kernels/Wrappers.cuh:75-99);
 so do the rows whose shrunken hash table overflows under the estimated

@@ -32,9 +32,26 @@ def test_batched_products_deterministic():
 
 
 def test_autotune_space_uses_config_keys():
+    """Every knob is a config key (per-bin knobs index a 7-entry array whose
+    0 means the gfx950 default); the first value is the default, and every
+    point of the space is a valid configuration."""
     c = ra.Config()
     for knob, values in autotune.SPACE.items():
-        assert values[0] == getattr(c, knob), knob  # the first value is the default
-        for v in values:
-            setattr(c, knob, v)
-            assert getattr(c, knob) == v
+        if isinstance(knob, str):
+            assert values[0] == getattr(c, knob), knob  # the first value is the default
+            for v in values:
+                setattr(c, knob, v)
+                assert getattr(c, knob) == v
+        else:
+            name, b = knob
+            assert values[0] == 0 and getattr(c, name)[b] == 0, knob
+            for v in values:
+                cur = {k: vv[0] for k, vv in autotune.SPACE.items()}
+                cur[knob] = v
+                cfg = ra.Config(**autotune._config(cur))
+                assert ra.ocean.lib.oc_config_check(ra.ocean.C.byref(cfg._c)) == 0, (knob, v)
+                assert (getattr(cfg, name)[b] == v) or v == 0
+                # the shape's tables fit its LDS (checked at handle creation)
+                h = ra.ocean.lib.oc_spgemm_create(ra.ocean.C.byref(cfg._c))
+                assert h, (knob, v, ra.ocean.lib.oc_last_error())
+                ra.ocean.lib.oc_spgemm_destroy(h)
