@@ -306,6 +306,21 @@ int oc_analysis(const oc_csr *dA, const oc_csr *dB, uint32_t *num_products,
                 uint32_t *b_right, uint64_t *products_psum,
                 uint32_t *max_a_nnz, uint32_t *max_products, void *stream);
 
+/* oc_analysis without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, outputs
+ * and errors as oc_analysis, but the temporaries live in a 32 MiB
+ * per-device array that the library holds until the process ends (no
+ * hipMalloc/hipFree, no stream-ordered allocation, no memset nodes) and the
+ * call does not synchronize: results are ready when the stream reaches them.
+ * Operands that need more than 32 MiB of temporaries (about 16 B per row of
+ * B plus 8 B per 8192 rows of A) are rejected with OC_ERR_INVALID.  Calls
+ * that overlap on one device race on its array. */
+int oc_analysis_noalloc(const oc_csr *dA, const oc_csr *dB, uint32_t *num_products,
+                        uint32_t *max_b_len, uint32_t *avg_b_len, uint32_t *b_left,
+                        uint32_t *b_right, uint64_t *products_psum,
+                        uint32_t *max_a_nnz, uint32_t *max_products, void *stream);
+
 /* Exclusive prefix sum of n u32 values into n+1 u64 values (out[n] is the
  * total).  Replaces: cub::DeviceScan::ExclusiveSum (kernels/SpGEMM.cuh:402,
  * 488, 978, 1147, 1221, 1441, 1622). */

@@ -118,6 +118,7 @@ def _load():
         "oc_spgemm_set_estimates": (I, [P, P]),
         "oc_get_tables": (I, [I, P]),
         "oc_analysis": (I, [P, P, P, P, P, P, P, P, P, P, P]),
+        "oc_analysis_noalloc": (I, [P, P, P, P, P, P, P, P, P, P, P]),
         "oc_scan_u32": (I, [P, P, U64, P]),
         "oc_murmur3_cols": (I, [P, P, U64, P]),
         "oc_hll_build": (I, [I, P, P, P]),
