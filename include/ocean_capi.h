@@ -231,8 +231,9 @@ int oc_spgemm_graph_launch(oc_spgemm *h, void *stream);
  * oc_spgemm_graph_launch_full launches that graph.  dC keeps the planned
  * capacity (the planned nnz).  After a run, oc_spgemm_full_status gives its
  * check bits: 1 nnz(C) beyond the capacity, 2 an A row longer than the
- * planned longest, 4 a symbolic table overflow, 8 a numeric recount; with
- * any bit set, dC's contents are not the product. */
+ * planned longest, 8 a numeric recount; with any bit set, dC's contents
+ * are not the product.  Rows whose (estimated-compaction) symbolic table
+ * fills up are recounted on the device, inside the same launch. */
 int oc_spgemm_full_replay(oc_spgemm *h, void *stream);
 int oc_spgemm_capture_full(oc_spgemm *h);
 int oc_spgemm_graph_launch_full(oc_spgemm *h, void *stream);

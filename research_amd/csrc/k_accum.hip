@@ -1441,7 +1441,10 @@ uint32_t rank_wide_log(uint32_t bs) {
 }
 
 int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row) {
-  const uint32_t lw = rank_wide_log(bs);
+  return wide_starts_for_log(b_cols, rank_wide_log(bs), entries_per_row);
+}
+
+int wide_starts_for_log(uint64_t b_cols, uint32_t lw, size_t *entries_per_row) {
   const uint64_t tiles = b_cols ? ((b_cols - 1) >> lw) + 1 : 1;
   if (tiles + 1 > kWideStarts) return 0;
   *entries_per_row = kWideStarts;
@@ -1652,9 +1655,6 @@ int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
                      void *bm, const RowSel &sel) {
   if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
-  // the widest bin's 1024-thread rows: flat product enumeration (k_rank.hip)
-  if (dense_bin == 6 && t.block[6] == 1024)
-    return launch_sym_flat(t.dense_sym[6], a, rows, nrows, nnz, s, bmoff, bm, sel);
   // whole 16-byte groups from column left & ~127 over the widest span
   const size_t lds = 256 + ((size_t)(t.dense_sym[dense_bin] + 126) / 128 + 1) * 16;
   KArgs<float> k = kargs<float>(a);

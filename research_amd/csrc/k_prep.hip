@@ -52,9 +52,8 @@ static Tables derive_tables(const int32_t *conc_in, const int32_t *block_in) {
     t.block[b] = block;
     t.lds[b] = l;
     t.hash_sym[b] = ((l - 256) / 4) & ~7u;
-    // dense bitmap over a span (plus one word of alignment slack); the
-    // widest bin's flat kernel also keeps 9216 B of segments and scan slots
-    t.dense_sym[b] = b == 0 ? 0 : (l - 512 - (b == 6 ? 9216u : 0u)) * 8;
+    // dense bitmap over a span (plus one word of alignment slack)
+    t.dense_sym[b] = b == 0 ? 0 : (l - 512) * 8;
     // numeric hash: keys (4 B) + FP64 sums (8 B) + sort buffers (4 B per
     // slot) + 1280 B header/digit totals + 512 B of radix counters per wave
     t.hash_num[b] = b == 0 ? 0 : ((l - 1312 - nw * 512) / 16) & ~7u;
@@ -295,6 +294,17 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
 int launch_check_gt(const void *x, int is64, uint64_t limit, uint32_t bit, uint32_t *err,
                     hipStream_t s) {
   hipLaunchKernelGGL(k_check_gt, dim3(1), dim3(64), 0, s, x, is64, limit, bit, err);
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+__global__ void k_count_bounds(const uint32_t *count, uint32_t *bounds) {
+  if (threadIdx.x == 0) {
+    bounds[0] = 0;
+    bounds[1] = *count;
+  }
+}
+int launch_count_bounds(const uint32_t *count, uint32_t *bounds, hipStream_t s) {
+  hipLaunchKernelGGL(k_count_bounds, dim3(1), dim3(64), 0, s, count, bounds);
   LAUNCH_CHECK();
   return OC_OK;
 }

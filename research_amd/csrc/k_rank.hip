@@ -340,12 +340,12 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
         ts_cur = tsv(my_k, t_begin);
         ts_end = tsv(my_k, t_begin + 1);
       }
-      // stored words of the next tile are loaded while the current one is
-      // summed (the first tile's here)
+      // (loading the next tile's stored words during the current tile's sums
+      // measured slower: the block barriers wait for the loads)
       uint32_t wv[4];
-      load_words((uint64_t)base_r + ((uint64_t)t_begin << lw), wv);
       for (uint32_t t = t_begin; t <= t_end; ++t) {
         const uint64_t wbase = (uint64_t)base_r + ((uint64_t)t << lw);
+        load_words(wbase, wv);
         const uint32_t pc = __popc(wv[0]) + __popc(wv[1]) + __popc(wv[2]) + __popc(wv[3]);
         uint32_t P = 0, ne = 0, m = 0;
         if (one_chunk) {
@@ -361,7 +361,6 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
           if (tid == 0) cex[ne] = P;
           if (stored) {
             write_words(wv, ew);
-            if (t + 1 <= t_end) load_words(wbase + (1ull << lw), wv);
           }
           ts_cur = ts_end;
           ts_end = nxt;
@@ -375,7 +374,6 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
           uint32_t ew, d1, d2, t1, t2;
           scan3(pc, 0u, 0u, ew, d1, d2, m, t1, t2);
           write_words(wv, ew);  // published by the first chunk's barrier
-          if (t + 1 <= t_end) load_words(wbase + (1ull << lw), wv);
         } else {
           for (uint32_t c0 = a0; c0 < a1; c0 += BS) {
             setup_chunk(c0, min(BS, a1 - c0), t, P, ne);
@@ -462,164 +460,6 @@ int launch_num_rank1(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uin
 
 uint32_t rank1_wide_log(uint32_t bs) {
   return bs == 512 ? R1Shape<512>::MAXLOG : R1Shape<1024>::MAXLOG;
-}
-
-// ---------------------------------------------------- flat dense symbolic
-// Exact row nnz from an LDS bitmap over [left & ~127, right] (the role of
-// denseSymbolicKernel, kernels/AccumulatorDense.cuh:43-126) for the widest
-// dense symbolic bin.  The row's products are enumerated flat, as in the
-// rank tiles: A entries in chunks of 1024, their (non-empty) B rows
-// compacted into segments, each wave a contiguous range of positions with
-// the ballot-window owners, so every lane marks the same number of
-// columns however uneven the B rows are (lane groups per A entry left
-// about half the lanes idle on R-MAT's long rows).  Rows with
-// bmoff[r] != ~0 store their bitmap as k_sym_dense does.
-constexpr uint32_t kFlatEntryBytes = 8704;  // cex[1028] + cq[1024] + pad
-
-template <int BS>
-__device__ __forceinline__ void sym_flat_row(KArgs<float> a, uint32_t r,
-                                             uint32_t *__restrict__ nnz,
-                                             const uint64_t *__restrict__ bmoff,
-                                             uint4 *__restrict__ bm) {
-  static_assert(BS == kR1Block, "the flat symbolic kernel runs 1024-thread workgroups");
-  constexpr uint32_t U = kR1U;
-  extern __shared__ __align__(16) uint8_t smem[];
-  uint32_t *scr = reinterpret_cast<uint32_t *>(smem);  // two 64-entry scan areas
-  uint32_t *cex = reinterpret_cast<uint32_t *>(smem + 512);
-  uint32_t *cq = cex + BS + 4;
-  uint32_t *bits = reinterpret_cast<uint32_t *>(smem + 512 + kFlatEntryBytes);
-  const uint32_t tid = threadIdx.x, lane = lane_id();
-  const uint32_t wid = __builtin_amdgcn_readfirstlane(tid / kWave);
-  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
-  const uint32_t lo = a.left[r] & ~127u, hi = a.right[r];
-  if (a1 == a0 || a.products[r] == 0 || hi < a.left[r]) {
-    if (tid == 0) nnz[r] = 0;
-    return;
-  }
-  const uint64_t bo = bmoff ? bmoff[r] : ~0ull;
-  const uint32_t n4 = ((hi - lo) >> 7) + 1;
-  uint4 *bits4 = reinterpret_cast<uint4 *>(bits);
-  for (uint32_t i = tid; i < n4; i += BS) bits4[i] = uint4{0, 0, 0, 0};
-  uint32_t sp = 0;
-  for (uint32_t c0 = a0; c0 < a1; c0 += BS) {
-    const uint32_t nc = min(BS, a1 - c0);
-    uint32_t len = 0, st = 0;
-    if (tid < nc) {
-      const uint32_t k = a.a_ci[c0 + tid];
-      st = a.b_rp[k];
-      len = a.b_rp[k + 1] - st;
-    }
-    uint32_t ex, ej, d, P, ne, t3;
-    block_excl_sum3<16>(len, len != 0u ? 1u : 0u, 0u, scr + 64 * sp, ex, ej, d, P, ne, t3);
-    sp ^= 1u;
-    if (len) {
-      cex[ej] = ex;
-      cq[ej] = st - ex;
-    }
-    if (tid == 0) cex[ne] = P;
-    __syncthreads();  // segments (and, on the first chunk, the cleared bitmap)
-    const uint32_t per = ((P + 15) / 16 + 63) & ~63u;
-    const uint32_t beg = min(P, wid * per), end = min(P, beg + per);
-    if (beg < end) {
-      uint32_t e_base;
-      {
-        const uint32_t S = (ne + 63) / 64;
-        const uint32_t i1 = lane * S;
-        const uint64_t b1 = __ballot(i1 < ne && cex[min(i1, ne - 1)] <= beg);
-        const uint32_t L1 = 63 - __clzll((long long)b1);
-        const uint32_t i2 = L1 * S + lane;
-        const uint64_t b2 = __ballot(lane < S && i2 < ne && cex[min(i2, ne - 1)] <= beg);
-        e_base = L1 * S + (63 - __clzll((long long)b2));
-      }
-      uint32_t nxw = e_base + 1 + lane < ne ? cex[e_base + 1 + lane] : kEmpty;
-      for (uint32_t pb = beg; pb < end; pb += 64 * U) {
-        uint32_t own[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-          const uint32_t p0 = pb + 64 * u;
-          own[u] = 0;
-          if (p0 < end) {
-            const uint32_t p = p0 + lane;
-            uint64_t mlo = __ballot(nxw <= p0);
-            if ((uint32_t)__builtin_amdgcn_readlane((int)nxw, kWave - 1) <= p0 + 63) {
-              e_base += (uint32_t)__popcll(mlo);
-              nxw = e_base + 1 + lane < ne ? cex[e_base + 1 + lane] : kEmpty;
-              mlo = 0;
-            }
-            uint64_t cand = __ballot(nxw <= p0 + 63) & ~mlo;
-            uint32_t ow = e_base + (uint32_t)__popcll(mlo);
-            while (cand) {
-              const int i = __builtin_ctzll(cand);
-              cand &= cand - 1;
-              ow += (uint32_t)__builtin_amdgcn_readlane((int)nxw, i) <= p ? 1u : 0u;
-            }
-            own[u] = ow;
-          }
-        }
-        uint32_t col[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-          const uint32_t p = pb + 64 * u + lane;
-          col[u] = p < end ? a.b_ci[cq[own[u]] + p] : kEmpty;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-          if (col[u] != kEmpty) {
-            const uint32_t x = col[u] - lo;
-            atomicOr(&bits[x >> 5], 1u << (x & 31));
-          }
-      }
-    }
-    __syncthreads();  // marks done; the next chunk rewrites the segments
-  }
-  uint32_t cnt = 0;
-  for (uint32_t i = tid; i < n4; i += BS) {
-    const uint4 v = bits4[i];
-    cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
-    if (bo != ~0ull) bm[bo + i] = v;
-  }
-  uint32_t ec, d1, d2, tot, t1, t2;
-  block_excl_sum3<16>(cnt, 0u, 0u, scr + 64 * sp, ec, d1, d2, tot, t1, t2);
-  if (tid == 0) nnz[r] = tot;
-}
-
-template <int BS>
-__global__ __launch_bounds__(BS) void k_sym_flat(KArgs<float> a, const uint32_t *__restrict__ rows,
-                                                 const uint32_t *__restrict__ bounds,
-                                                 uint32_t *__restrict__ nnz,
-                                                 const uint64_t *__restrict__ bmoff,
-                                                 uint4 *__restrict__ bm) {
-  if (!bounds) {
-    sym_flat_row<BS>(a, rows[blockIdx.x], nnz, bmoff, bm);
-    return;
-  }
-  // persistent: the rows [bounds[0], bounds[1]) of the list
-  const uint32_t b1 = bounds[1];
-  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
-    sym_flat_row<BS>(a, rows[i], nnz, bmoff, bm);
-    __syncthreads();  // the next row re-initialises the LDS
-  }
-}
-
-uint32_t sym_flat_entry_bytes() { return kFlatEntryBytes; }
-
-int launch_sym_flat(uint32_t span_bits, const AccArgs &a, const uint32_t *rows, uint32_t nrows,
-                    uint32_t *nnz, hipStream_t s, const uint64_t *bmoff, void *bm,
-                    const RowSel &sel) {
-  if (nrows == 0 && !sel.bounds) return OC_OK;
-  const Tables &t = tables();
-  const size_t lds = 512 + kFlatEntryBytes + ((size_t)(span_bits + 126) / 128 + 1) * 16;
-  if (lds > t.lds_max_block) {
-    set_error("flat dense symbolic kernel needs %zu B of LDS, device allows %u", lds,
-              t.lds_max_block);
-    return OC_ERR_INVALID;
-  }
-  OC_TRY(set_lds(k_sym_flat<kR1Block>, lds));
-  const uint32_t grid = sel.bounds ? std::max<uint32_t>(1, t.num_cu) : nrows;
-  hipLaunchKernelGGL(k_sym_flat<kR1Block>, dim3(grid), dim3(kR1Block), lds, s, kargs<float>(a),
-                     rows, sel.bounds, nnz, bmoff, (uint4 *)bm);
-  LAUNCH_CHECK();
-  return OC_OK;
 }
 
 // ------------------------------------------------------ stored row bitmaps

@@ -70,6 +70,8 @@ int launch_analysis(const DevCsr &A, const DevCsr &B, const RowInfo &ri,
 int launch_check_gt(const void *x, int is64, uint64_t limit, uint32_t bit, uint32_t *err,
                     hipStream_t s);
 int launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
+// bounds = {0, *count}: the row range of a device-counted list
+int launch_count_bounds(const uint32_t *count, uint32_t *bounds, hipStream_t s);
 size_t scan_temp_bytes(uint64_t n);
 int launch_scan_u32(const uint32_t *in, uint64_t *out, uint64_t n, void *temp,
                     hipStream_t s);
@@ -204,19 +206,14 @@ int launch_num_tiles(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uin
 // entries per B row, built by launch_tile_starts at rank_wide_log(bs)).
 uint32_t rank_wide_log(uint32_t bs);
 int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row);
+// the same for a table of 2^lw-column tiles
+int wide_starts_for_log(uint64_t b_cols, uint32_t lw, size_t *entries_per_row);
 // bbase (optional, per B row): the first column of the row's block in a
 // block-diagonal product; the tile tables are then built from the same bases.
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
                     const uint32_t *ts, uint32_t T1, const uint32_t *tsw, const uint32_t *bbase,
                     uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s,
                     const RowSel &sel = RowSel{});
-// Flat dense symbolic kernel (k_rank.hip) for the widest dense symbolic bin
-// (1024 threads): spans up to span_bits, LDS = 512 + sym_flat_entry_bytes()
-// + the bitmap; bitmap store as launch_sym_dense.
-uint32_t sym_flat_entry_bytes();
-int launch_sym_flat(uint32_t span_bits, const AccArgs &a, const uint32_t *rows, uint32_t nrows,
-                    uint32_t *nnz, hipStream_t s, const uint64_t *bmoff, void *bm,
-                    const RowSel &sel);
 // One-gather rank tiles (k_rank.hip), 1024 or 512 threads (bs), tiles up
 // to 2^rank1_wide_log(bs) columns; tsw only when built at that width.  Rows
 // with a stored bitmap (bmoff[r] != ~0, no block bases) gather each

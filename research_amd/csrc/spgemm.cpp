@@ -246,6 +246,7 @@ struct oc_spgemm {
     uint64_t cap_nnz = 0;               // C's capacity (the planned nnz)
     uint32_t tsw_log = 0, tsw_per = 0;  // compact tile-start table shape
     BinParams sym_bp{}, num_bp{};
+    bool est_symbolic = false;          // shrunken symbolic tables (recount overflow rows)
     hipGraph_t graph_full = nullptr;
     hipGraphExec_t exec_full = nullptr;
     uint32_t tile_logw = 0, tile_t1 = 0, max_a_nnz = 0;
@@ -461,9 +462,10 @@ struct Run {
       // the one-gather kernel runs 1024- or 512-thread workgroups
       const uint32_t bs = mode == 3 ? (h->cfg.tile_block == 512 ? 512u : 1024u)
                                     : (uint32_t)h->cfg.tile_block;
-      if (mode >= 2 && wide_starts_plan(width, bs, &per)) {
+      const uint32_t wlog = mode == 3 ? rank1_wide_log(bs) : rank_wide_log(bs);
+      if (mode >= 2 && wide_starts_for_log(width, wlog, &per)) {
         OC_TRY(ar.get(&tile_tsw, B.rows * per));
-        tsw_log = rank_wide_log(bs);
+        tsw_log = wlog;
         tsw_per = (uint32_t)per;
         OC_TRY(launch_tile_starts(B, tsw_log, tsw_per, tile_tsw, s, block_base));
       }
@@ -1140,8 +1142,8 @@ int Run::run_precise() {
     p.nnz = nnz;
     p.ovf = ovf;
     p.cap_nnz = C->nnz;
-    p.sym_bp = bp;
-    p.sym_bp.use_compaction = 0;  // whole-path replays use full tables (no overflow)
+    p.sym_bp = bp;  // with the sampled compaction when the run used it
+    p.est_symbolic = d.use_est_symbolic;
     p.num_bp = nb;
     if (!p.err) OC_TRY(ar.get(&p.err, 64));
     p.tile_logw = tile_logw;
@@ -1665,9 +1667,13 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
     if (rc == OC_OK && h->cfg.use_segmented_workflow && h->d.ana1_type != 2)
       rc = r.segmentation();
     if (rc == OC_OK && h->keep_plan) {
+      // the sampled compaction decides the symbolic tables, as in a plain
+      // run of the HyperLogLog-sampled path; the plan itself is always the
+      // precise workflow
+      if (h->d.ana1_type == 1) rc = r.sampling();
       h->d.ana1_type = 1;  // reported as the precise workflow
       h->d.ana2_type = 0;
-      rc = r.run_precise();
+      if (rc == OC_OK) rc = r.run_precise();
     } else if (rc == OC_OK) {
       const bool learned = h->model != nullptr && h->d.ana1_type == 1 && h->force_workflow != 0;
       bool use_est = false;
@@ -1782,7 +1788,7 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
 // Check bits of a whole-path replay (oc_spgemm_full_status).
 constexpr uint32_t kFullCapacity = 1;   // nnz(C) beyond the planned capacity
 constexpr uint32_t kFullLongA = 2;      // an A row longer than the planned longest
-constexpr uint32_t kFullSymOvf = 4;     // a symbolic table overflowed
+constexpr uint32_t kFullSymOvf = 4;     // (unused: symbolic overflow rows are recounted)
 constexpr uint32_t kFullNumOvf = 8;     // a numeric row needed a recount
 
 // The whole precise path of a kept plan on stream s, from the analysis to
@@ -1837,15 +1843,22 @@ int full_replay_impl(oc_spgemm *h, hipStream_t s) {
   // symbolic: binning, then every bin persistent over its device range
   OC_TRY(launch_binning(p.sym_bp, m, p.bor1, p.brows1, p.boff1, p.bin_tmp, s));
   uint32_t *ovf_rows = p.ovf + 8;
-  for (int b = 0; b <= 5; ++b)
-    OC_TRY(launch_sym_hash(b, a, p.brows1, 0, t.hash_sym[b], p.nnz, ovf_rows, p.ovf, s,
-                           RowSel{p.boff1 + b, nullptr}));
+  for (int b = 0; b <= 5; ++b) {
+    const uint32_t S = t.hash_sym[b];
+    OC_TRY(launch_sym_hash(b, a, p.brows1, 0, p.est_symbolic ? S - S / 8 : S, p.nnz, ovf_rows,
+                           p.ovf, s, RowSel{p.boff1 + b, nullptr}));
+  }
   OC_TRY(launch_sym_window(a, p.brows1, 0, p.max_a_nnz, p.nnz, p.win_scratch, s,
                            RowSel{p.boff1 + 6, err}));
   for (int b = 8; b <= 13; ++b)
     OC_TRY(launch_sym_dense(b - 7, a, p.brows1, 0, p.nnz, s, nullptr, nullptr,
                             RowSel{p.boff1 + b, nullptr}));
-  OC_TRY(launch_check_gt(p.ovf, 0, 0, kFullSymOvf, err, s));
+  // rows whose table filled up are recounted by the windowed kernel over
+  // the device-side overflow list [0, ovf[0])
+  uint32_t *ovf_bounds = err + 8;
+  OC_TRY(launch_count_bounds(p.ovf, ovf_bounds, s));
+  OC_TRY(launch_sym_window(a, ovf_rows, 0, p.max_a_nnz, p.nnz, p.win_scratch, s,
+                           RowSel{ovf_bounds, err}));
   // C's offsets, and its capacity against the new nnz
   OC_TRY(launch_scan_u32(p.nnz, (uint64_t *)p.o.off, m, p.scan_tmp, s));
   OC_TRY(launch_check_gt(p.o.off + m, 1, p.cap_nnz, kFullCapacity, err, s));

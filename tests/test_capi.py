@@ -37,8 +37,7 @@ def test_tables_gfx950():
     # 160 KB of LDS per CU: per-bin LDS budgets and the table sizes they give
     assert t["lds_bytes"] == [5120, 5120, 10240, 20480, 40960, 81920, 163840]
     assert t["hash_sym"][5] == (81920 - 256) // 4 // 8 * 8
-    # the widest dense symbolic bin leaves 9216 B for the flat kernel's segments
-    assert t["dense_sym"][6] == (163840 - 512 - 9216) * 8 and t["dense_sym"][0] == 0
+    assert t["dense_sym"][6] == (163840 - 512) * 8 and t["dense_sym"][0] == 0
     assert t["hash_num"][0] == 0 and t["dense_num"][6] == 0xFFFFFFFF
     assert all(t["hash_num"][b] < t["hash_num"][b + 1] for b in range(1, 6))
 

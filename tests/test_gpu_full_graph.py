@@ -152,3 +152,29 @@ def test_full_graph_batch(gpu):
                           f"batch launch {it} product {i}")
         log(f"batch whole-path launch {it}: structure of 40 products identical, "
             f"4 products against the oracle")
+
+
+def test_full_graph_estimated_symbolic(gpu):
+    """A product whose sampled compaction turns on the shrunken symbolic
+    tables (the banded matrix with spread rows of test_gpu_wide.py): in the
+    captured whole path the rows that fill their tables are recounted by the
+    windowed kernel over the device-side overflow list, and the product of
+    permuted inputs still equals the oracle's."""
+    from test_gpu_wide import est_sym_matrix
+    log("case: whole-path graph with estimated symbolic tables")
+    M = est_sym_matrix(gpu)
+    dM = M.upload()
+    sp = gpu.SpGEMM(gpu.Config())
+    dC = sp.plan(dM, dM)
+    st = sp.get_stats()
+    assert st["decision"]["estimated_symbolic"] is True
+    assert st["decision"]["symbolic_overflow_rows"] > 0
+    sp.capture_full()
+    for it in range(2):
+        X = M if it == 0 else _permuted(gpu, M, 77)
+        _overwrite(gpu, dM, X)
+        sp.graph_launch_full()
+        assert sp.full_status() == 0
+        worst, _ = check_product(X, X, dC.download(), TOL64, f"estimated symbolic, launch {it}")
+        gpu.SpGEMM(gpu.Config()).run(dM, dM).free()  # eager work between launches
+        log(f"estimated symbolic whole path, launch {it}: err {worst:.2e}")
