@@ -41,35 +41,77 @@ struct BatchDesc {
   uint32_t col_base, pad;
 };
 
-// One workgroup per input matrix.  VALUES_ONLY: refresh the values.
-template <bool VALUES_ONLY>
-__global__ __launch_bounds__(256) void k_batch_concat(const BatchDesc *__restrict__ d,
-                                                      uint32_t count, int vbytes,
-                                                      uint32_t *__restrict__ rp_out,
-                                                      uint32_t *__restrict__ ci_out,
-                                                      uint8_t *__restrict__ v_out,
-                                                      uint64_t total_rows, uint64_t total_nnz) {
-  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
-    const BatchDesc x = d[i];
-    const uint32_t r0 = x.rp[0];
-    if (!VALUES_ONLY) {
-      for (uint64_t j = threadIdx.x; j < x.rows; j += blockDim.x)
-        rp_out[x.row_base + j] = (uint32_t)(x.nnz_base + (x.rp[j] - r0));
-      for (uint64_t e = threadIdx.x; e < x.nnz; e += blockDim.x)
-        ci_out[x.nnz_base + e] = x.ci[r0 + e] + x.col_base;
+// The batch's operands as one block-diagonal CSR.  The rows and the entries
+// of all products are cut into chunks of kConcatChunk and each workgroup
+// copies one chunk, so a large product is spread over many workgroups (one
+// workgroup per product left the largest ones to a single CU).  A thread
+// finds the product of its first element by a binary search over the
+// products' first rows (entries), then steps forward over the short products
+// a chunk may hold.  VALUES_ONLY: refresh the values.
+constexpr uint32_t kConcatChunk = 8192;
+
+template <bool ENTRIES>
+__device__ __forceinline__ uint64_t desc_base(const BatchDesc &x) {
+  return ENTRIES ? x.nnz_base : x.row_base;
+}
+
+// the last product whose first row (entry) is at or below e; d[0] starts at 0
+template <bool ENTRIES>
+__device__ uint32_t product_of(const BatchDesc *__restrict__ d, uint32_t count, uint64_t e) {
+  uint32_t lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (desc_base<ENTRIES>(d[mid]) <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool ENTRIES, bool VALUES_ONLY, typename V>
+__device__ void concat_chunk(const BatchDesc *__restrict__ d, uint32_t count, uint64_t c,
+                             uint64_t n, uint32_t *__restrict__ rp_out,
+                             uint32_t *__restrict__ ci_out, V *__restrict__ v_out) {
+  const uint64_t x0 = c * kConcatChunk + threadIdx.x;
+  const uint64_t x1 = min(n, (c + 1) * (uint64_t)kConcatChunk);
+  if (x0 >= x1) return;
+  uint32_t i = product_of<ENTRIES>(d, count, x0);
+  BatchDesc D = d[i];
+  uint64_t next = i + 1 < count ? desc_base<ENTRIES>(d[i + 1]) : ~0ull;
+  uint32_t r0 = D.rp[0];
+  for (uint64_t x = x0; x < x1; x += 256) {
+    if (x >= next) {
+      do {
+        ++i;
+        next = i + 1 < count ? desc_base<ENTRIES>(d[i + 1]) : ~0ull;
+      } while (x >= next);
+      D = d[i];
+      r0 = D.rp[0];
     }
-    if (vbytes == 8) {
-      const double *src = (const double *)x.v + r0;
-      double *dst = (double *)v_out + x.nnz_base;
-      for (uint64_t e = threadIdx.x; e < x.nnz; e += blockDim.x) dst[e] = src[e];
+    if (!ENTRIES) {
+      rp_out[x] = (uint32_t)(D.nnz_base + (D.rp[x - D.row_base] - r0));
     } else {
-      const float *src = (const float *)x.v + r0;
-      float *dst = (float *)v_out + x.nnz_base;
-      for (uint64_t e = threadIdx.x; e < x.nnz; e += blockDim.x) dst[e] = src[e];
+      const uint64_t src = r0 + (x - D.nnz_base);
+      if (!VALUES_ONLY) ci_out[x] = D.ci[src] + D.col_base;
+      v_out[x] = reinterpret_cast<const V *>(D.v)[src];
     }
   }
+}
+
+// blocks [0, row_chunks): rows (unless VALUES_ONLY); the rest: entries
+template <bool VALUES_ONLY, typename V>
+__global__ __launch_bounds__(256) void k_batch_concat(const BatchDesc *__restrict__ d,
+                                                      uint32_t count, uint32_t row_chunks,
+                                                      uint32_t *__restrict__ rp_out,
+                                                      uint32_t *__restrict__ ci_out,
+                                                      V *__restrict__ v_out,
+                                                      uint64_t total_rows, uint64_t total_nnz) {
   if (!VALUES_ONLY && blockIdx.x == 0 && threadIdx.x == 0)
     rp_out[total_rows] = (uint32_t)total_nnz;
+  if (!VALUES_ONLY && blockIdx.x < row_chunks)
+    concat_chunk<false, VALUES_ONLY, V>(d, count, blockIdx.x, total_rows, rp_out, ci_out, v_out);
+  else
+    concat_chunk<true, VALUES_ONLY, V>(d, count, blockIdx.x - (VALUES_ONLY ? 0u : row_chunks),
+                                       total_nnz, rp_out, ci_out, v_out);
 }
 
 // Product i of a batched result as its own CSR (u64 offsets from 0, column
@@ -151,16 +193,26 @@ struct oc_batch {
 
 static int concat(const oc_batch *b, const BatchDesc *d, const oc_csr &M, uint64_t total_nnz,
                   bool values_only, hipStream_t s) {
-  const int vb = (int)vsize(b->vtype);
-  const uint32_t grid = std::min<uint32_t>(b->count, 8192);
-  if (values_only)
-    hipLaunchKernelGGL(k_batch_concat<true>, dim3(grid), dim3(256), 0, s, d, b->count, vb,
-                       (uint32_t *)M.row_offsets, M.col_ids, (uint8_t *)M.values, M.rows,
-                       total_nnz);
-  else
-    hipLaunchKernelGGL(k_batch_concat<false>, dim3(grid), dim3(256), 0, s, d, b->count, vb,
-                       (uint32_t *)M.row_offsets, M.col_ids, (uint8_t *)M.values, M.rows,
-                       total_nnz);
+  const uint64_t rc = values_only ? 0 : (M.rows + kConcatChunk - 1) / kConcatChunk;
+  const uint64_t ec = (total_nnz + kConcatChunk - 1) / kConcatChunk;
+  if (rc + ec > 0x7fffffffull) {
+    set_error("batched operand of %llu rows and %llu entries is too large",
+              (unsigned long long)M.rows, (unsigned long long)total_nnz);
+    return OC_ERR_INVALID;
+  }
+  const dim3 grid((uint32_t)std::max<uint64_t>(1, rc + ec));
+  uint32_t *rp = (uint32_t *)M.row_offsets;
+#define GO(VO, V)                                                                       \
+  hipLaunchKernelGGL((k_batch_concat<VO, V>), grid, dim3(256), 0, s, d, b->count,        \
+                     (uint32_t)rc, rp, M.col_ids, (V *)M.values, (uint64_t)M.rows, total_nnz)
+  if (vsize(b->vtype) == 8) {
+    if (values_only) GO(true, double);
+    else GO(false, double);
+  } else {
+    if (values_only) GO(true, float);
+    else GO(false, float);
+  }
+#undef GO
   LAUNCH_CHECK();
   return OC_OK;
 }

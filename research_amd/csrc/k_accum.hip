@@ -740,7 +740,8 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows,
     uint32_t *counter, uint32_t W, KOut<T> o, uint32_t *__restrict__ nnz_sym,
     uint32_t *__restrict__ scratch, uint32_t max_nA, const uint32_t *__restrict__ bounds,
-    const uint32_t *__restrict__ err) {
+    const uint32_t *__restrict__ err, const uint64_t *__restrict__ bmoff,
+    uint32_t *__restrict__ bm32) {
   if (err && *err) return;  // a capacity check of the run failed
   if (bounds) {             // rows [bounds[0], bounds[1]) of the list
     rows += bounds[0];
@@ -780,6 +781,18 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
     while (G < 64 && (uint32_t)G < per) G <<= 1;
     const int ngroups = kWinBlock / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
     const uint64_t off = SYM ? 0 : o.off[r];
+    // SYM: a row with a stored bitmap (bmoff[r] != ~0) gets each window's
+    // words, in 32-bit words from column left & ~127 (the layout of
+    // k_sym_dense); the words outside [lo, hi] of its first and last
+    // 128-column groups are zeroed here
+    uint32_t *sw = nullptr;
+    const uint32_t lo4 = left & ~127u;
+    if (SYM && bmoff && bmoff[r] != ~0ull) {
+      sw = bm32 + 4 * bmoff[r];
+      if (threadIdx.x < (lo - lo4) / 32) sw[threadIdx.x] = 0u;
+      const uint32_t last = (hi - lo4) / 32, nw4 = 4 * (((hi - lo4) >> 7) + 1);
+      if (threadIdx.x >= 4 && threadIdx.x - 4 < nw4 - 1 - last) sw[last + 1 + threadIdx.x - 4] = 0u;
+    }
     for (uint32_t w = 0; w < nwin; ++w) {
       const uint64_t w0 = (uint64_t)lo + (uint64_t)w * W;
       const uint64_t w1 = min((uint64_t)hi + 1, w0 + W);
@@ -813,6 +826,10 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
       uint32_t wtotal;
       const uint32_t base = block_excl_sum(mycnt, hdr + 8, &wtotal);
       if (SYM) {
+        if (sw) {
+          uint32_t *dst = sw + (uint32_t)((w0 - lo4) / 32);
+          for (uint32_t i = b0; i < b1; ++i) dst[i] = bits[i];
+        }
         for (uint32_t i = b0; i < b1; ++i) bits[i] = 0;
       } else {
         uint64_t pos = off + total + base;
@@ -1060,7 +1077,8 @@ template <typename T, int BS>
 __global__ __launch_bounds__(BS, 4) void k_num_rank(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
     uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1,
-    const uint32_t *__restrict__ tsw, const uint32_t *__restrict__ bbase, uint32_t cap,
+    const uint32_t *__restrict__ tsw, uint32_t wper, const uint32_t *__restrict__ bbase,
+    uint32_t cap,
     uint32_t pcap, KOut<T> o, const uint32_t *__restrict__ bounds,
     const uint32_t *__restrict__ err) {
   if (err && *err) return;  // a capacity check of the run failed
@@ -1150,7 +1168,7 @@ __global__ __launch_bounds__(BS, 4) void k_num_rank(
     // row's tiles share stays cached between them).
     const bool wide = tsw != nullptr && lw == S::MAXLOG;
     auto tsv = [&](uint32_t k, uint32_t tt) {
-      if (wide) return tsw[(uint64_t)k * kWideStarts + min(kWideStarts - 1, tt)];
+      if (wide) return tsw[(uint64_t)k * wper + min(wper - 1, tt)];
       return ts[(uint64_t)k * T1 + min(T1 - 1, tt << step)];
     };
     uint32_t ts_cur[E], ts_end[E];
@@ -1446,13 +1464,14 @@ int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row) {
 
 int wide_starts_for_log(uint64_t b_cols, uint32_t lw, size_t *entries_per_row) {
   const uint64_t tiles = b_cols ? ((b_cols - 1) >> lw) + 1 : 1;
-  if (tiles + 1 > kWideStarts) return 0;
-  *entries_per_row = kWideStarts;
+  if (tiles + 1 > kMaxWideStarts) return 0;
+  *entries_per_row = (size_t)tiles + 1;
   return 1;
 }
 
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, const uint32_t *bbase,
+                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, uint32_t wper,
+                    const uint32_t *bbase,
                     uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s,
                     const RowSel &sel) {
   if (nrows == 0 && !sel.bounds) return OC_OK;
@@ -1476,7 +1495,7 @@ int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint
     const size_t lds = fixed + (size_t)cap * 12;                                    \
     OC_TRY(set_lds(k_num_rank<T, B>, lds));                                         \
     hipLaunchKernelGGL((k_num_rank<T, B>), dim3(grid), dim3(B), lds, s, kargs<T>(a), \
-                       rows, nrows, counter, logT, ts, T1, tsw, bbase, cap, pcap,         \
+                       rows, nrows, counter, logT, ts, T1, tsw, wper, bbase, cap, pcap,   \
                        kout<T>(o), sel.bounds, sel.err);                                     \
   } while (0)
   if (a.A.vtype == OC_F64) {
@@ -1565,7 +1584,8 @@ static size_t window_lds(uint32_t W, bool sym) {
 template <typename T, bool SYM>
 static int window_go(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                      uint32_t max_a_nnz, const NumOut &o, uint32_t *nnz_sym,
-                     void *scratch, hipStream_t s, const RowSel &sel) {
+                     void *scratch, hipStream_t s, const RowSel &sel,
+                     const uint64_t *bmoff = nullptr, void *bm = nullptr) {
   if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
   const uint32_t W = SYM ? t.win_sym_cols : t.win_num_cols;
@@ -1583,17 +1603,17 @@ static int window_go(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
   KOut<T> ko = kout<T>(o);
   hipLaunchKernelGGL((k_window<T, SYM>), dim3(grid), dim3(kWinBlock), lds, s,
                      kargs<T>(a), rows, nrows, counter, W, ko, nnz_sym, cur,
-                     max_a_nnz, sel.bounds, sel.err);
+                     max_a_nnz, sel.bounds, sel.err, bmoff, (uint32_t *)bm);
   LAUNCH_CHECK();
   return OC_OK;
 }
 
 int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                       uint32_t max_a_nnz, uint32_t *nnz, void *scratch,
-                      hipStream_t s, const RowSel &sel) {
+                      hipStream_t s, const RowSel &sel, const uint64_t *bmoff, void *bm) {
   NumOut o;
   std::memset(&o, 0, sizeof(o));
-  return window_go<float, true>(a, rows, nrows, max_a_nnz, o, nnz, scratch, s, sel);
+  return window_go<float, true>(a, rows, nrows, max_a_nnz, o, nnz, scratch, s, sel, bmoff, bm);
 }
 
 int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
@@ -1621,11 +1641,20 @@ int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
   }
 
 // Workgroups of a persistent launch: as many as fit on the chip at once.
-static uint32_t persistent_grid(uint32_t bs, size_t lds) {
+// Persistent grids hold as many workgroups as the device keeps resident at
+// once for the kernel (its registers, LDS and waves), so that every
+// workgroup of the row-striding loop starts at launch.
+template <typename K>
+static uint32_t persistent_grid(K kernel, uint32_t bs, size_t lds) {
   const Tables &t = tables();
-  const uint32_t by_lds = lds ? (uint32_t)(t.lds_per_cu / lds) : 64u;
-  const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(2048 / bs, by_lds));
-  return std::max<uint32_t>(1, t.num_cu) * per_cu;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)bs, lds) !=
+          hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  return std::max<uint32_t>(1, t.num_cu) * (uint32_t)per_cu;
 }
 
 int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
@@ -1639,7 +1668,7 @@ int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
 #define GO(BS)                                                                \
   do {                                                                        \
     OC_TRY(set_lds(k_sym_hash<BS>, lds));                                     \
-    const uint32_t grid = sel.bounds ? persistent_grid(BS, lds) : nrows;      \
+    const uint32_t grid = sel.bounds ? persistent_grid(k_sym_hash<BS>, BS, lds) : nrows; \
     hipLaunchKernelGGL(k_sym_hash<BS>, dim3(grid), dim3(BS), lds, s, k, rows,  \
                        sel.bounds, S, std::min(limit, S), nnz, ovf_rows,      \
                        ovf_count);                                            \
@@ -1661,7 +1690,7 @@ int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
 #define GO(BS)                                                                 \
   do {                                                                         \
     OC_TRY(set_lds(k_sym_dense<BS>, lds));                                     \
-    const uint32_t grid = sel.bounds ? persistent_grid(BS, lds) : nrows;       \
+    const uint32_t grid = sel.bounds ? persistent_grid(k_sym_dense<BS>, BS, lds) : nrows; \
     hipLaunchKernelGGL(k_sym_dense<BS>, dim3(grid), dim3(BS), lds, s, k, rows,  \
                        sel.bounds, nnz, bmoff, (uint4 *)bm);                   \
   } while (0)
@@ -1685,7 +1714,7 @@ int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
 #define GO3(BS, T, E)                                                          \
   do {                                                                         \
     OC_TRY(set_lds(k_num_hash<BS, T, E>, L.bytes));                            \
-    const uint32_t grid = sel.bounds ? persistent_grid(BS, L.bytes) : nrows;   \
+    const uint32_t grid = sel.bounds ? persistent_grid(k_num_hash<BS, T, E>, BS, L.bytes) : nrows; \
     hipLaunchKernelGGL((k_num_hash<BS, T, E>), dim3(grid), dim3(BS), L.bytes,  \
                        s, kargs<T>(a), rows, sel.bounds, sel.err, L,           \
                        kout<T>(o), kb);                                        \
@@ -1722,7 +1751,7 @@ int launch_num_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
 #define GO3(BS, T, E)                                                           \
   do {                                                                          \
     OC_TRY(set_lds(k_num_dense<BS, T, E>, lds));                                \
-    const uint32_t grid = sel.bounds ? persistent_grid(BS, lds) : nrows;        \
+    const uint32_t grid = sel.bounds ? persistent_grid(k_num_dense<BS, T, E>, BS, lds) : nrows; \
     hipLaunchKernelGGL((k_num_dense<BS, T, E>), dim3(grid), dim3(BS), lds, s,   \
                        kargs<T>(a), rows, sel.bounds, sel.err, kout<T>(o));     \
   } while (0)

@@ -110,7 +110,8 @@ static inline int zero_u32(uint32_t *p, uint32_t n, hipStream_t s) {
   return OC_OK;
 }
 
-// Entries per B row of the compact tile-start table of the widest tile.
-constexpr uint32_t kWideStarts = 16;
+// Most entries per B row of the compact tile-start table of the widest
+// tile (its tiles + 1: 63 tiles of 2^17 columns reach 8.2M columns).
+constexpr uint32_t kMaxWideStarts = 64;
 
 }  // namespace oc

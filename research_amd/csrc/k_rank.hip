@@ -84,18 +84,89 @@ struct BoolC {
   static constexpr bool value = V;
 };
 
-// One 1024-thread workgroup per row, rows taken from an atomic counter.
-// Requires fewer than 2^31 products per row (the analysis stores them as
-// u32).  bm/bmoff: stored row bitmaps (uint4 groups from column
-// left & ~127; bmoff[r] = ~0 for rows without one); only used without
-// block bases.
+// Tile width of a row: the widest 2^lw (from 2^logT up to 2^maxlog) whose
+// tiles are expected to fill at most about 3/4 of the cap slots.
+__device__ __forceinline__ uint32_t r1_tile_log(uint32_t logT, uint32_t maxlog, uint64_t span,
+                                                uint64_t est, uint32_t cap) {
+  uint32_t lw = logT;
+  while (lw < maxlog && (span >> lw) >= 1 &&
+         est * (1ull << (lw + 1)) <= (uint64_t)cap * 3 / 4 * span)
+    ++lw;
+  return lw;
+}
+
+// Pieces of long rows.  A row with a stored bitmap and more than per_piece
+// products is cut into K pieces of consecutive tiles (K at most its tiles
+// and kR1MaxPieces), which different workgroups take: a piece finds the
+// outputs of the row before its first tile from the stored bitmap, so no
+// piece waits for another.  kp[i] = K of list entry i; all K pieces are
+// appended to `extra` as (i, piece) pairs, at most max_extra of them (a row
+// that does not fit stays whole).  The kernel takes the pieces before the
+// list, so the heaviest work starts first (taken after the list, the
+// pieces left a tail: R-MAT 2^20 went from 110.7 to 114.3 ms).
+constexpr uint32_t kR1MaxPieces = 64;
+
+template <int BS>
+__global__ __launch_bounds__(256) void k_r1_pieces(
+    const uint32_t *__restrict__ rows, uint32_t nrows, const uint32_t *__restrict__ products,
+    const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
+    const uint32_t *__restrict__ nnz_hint, const uint64_t *__restrict__ bmoff, uint32_t logT,
+    uint32_t cap, uint32_t per_piece, uint8_t *__restrict__ kp, uint32_t *__restrict__ extra,
+    uint32_t *n_extra, uint32_t max_extra) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nrows) return;
+  const uint32_t r = rows[i];
+  const uint32_t l = left[r], h = right[r];
+  const uint64_t prod = products[r];
+  uint32_t K = 1;
+  if (bmoff[r] != ~0ull && h >= l && prod > per_piece) {
+    const uint64_t span = (uint64_t)h - l + 1;
+    const uint64_t est = nnz_hint ? nnz_hint[r] : (prod < span ? prod : span);
+    const uint32_t lw = r1_tile_log(logT, R1Shape<BS>::MAXLOG, span, est, cap);
+    const uint64_t nt = (uint64_t)(h >> lw) - (l >> lw) + 1;
+    const uint64_t want = (prod + per_piece - 1) / per_piece;
+    K = (uint32_t)min(min(want, nt), (uint64_t)kR1MaxPieces);
+    if (K > 1) {
+      // reserve K extra entries, or none (lock-free: a failed swap means
+      // another row's reservation succeeded)
+      uint32_t cur = __hip_atomic_load(n_extra, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t base = ~0u;
+      while (cur + K <= max_extra) {
+        const uint32_t prev = atomicCAS(n_extra, cur, cur + K);
+        if (prev == cur) {
+          base = cur;
+          break;
+        }
+        cur = prev;
+      }
+      if (base == ~0u) {
+        K = 1;
+      } else {
+        for (uint32_t j = 0; j < K; ++j) {
+          extra[2 * (base + j)] = i;
+          extra[2 * (base + j) + 1] = j;
+        }
+      }
+    }
+  }
+  kp[i] = (uint8_t)K;
+}
+
+// One 1024-thread workgroup per row or row piece, taken from an atomic
+// counter: the pieces of k_r1_pieces first (when kp is given), then list
+// entries [0, nrows), skipping those of split rows.  Requires fewer than 2^31
+// products per row (the analysis stores them as u32).  bm/bmoff: stored
+// row bitmaps (uint4 groups from column left & ~127; bmoff[r] = ~0 for rows
+// without one); only used without block bases.
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void k_num_rank1(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t *counter,
     uint32_t logT, const uint32_t *__restrict__ ts, uint32_t T1,
-    const uint32_t *__restrict__ tsw, const uint32_t *__restrict__ bbase,
+    const uint32_t *__restrict__ tsw, uint32_t wper, const uint32_t *__restrict__ bbase,
     const uint4 *__restrict__ bm, const uint64_t *__restrict__ bmoff, uint32_t cap,
-    KOut<T> o, const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err) {
+    KOut<T> o, const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err,
+    const uint8_t *__restrict__ kp, const uint32_t *__restrict__ extra,
+    const uint32_t *__restrict__ n_extra) {
   if (err && *err) return;  // a capacity check of the run failed
   if (bounds) {             // rows [bounds[0], bounds[1]) of the list
     rows += bounds[0];
@@ -122,28 +193,40 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
     block_excl_sum3<NW>(x, y, z, scr + 64 * sp, ex, ey, ez, tx, ty, tz);
     sp ^= 1u;
   };
+  const uint32_t nex = kp ? *n_extra : 0u;
   __syncthreads();
   for (;;) {
-    if (tid == 0) hdr[0] = atomicAdd(counter, 1u);
+    if (tid == 0) {
+      const uint32_t c = atomicAdd(counter, 1u);
+      uint32_t i = ~0u, piece = 0;
+      if (c < nex) {
+        i = extra[2 * c];
+        piece = extra[2 * c + 1];
+      } else if (c - nex < nrows) {
+        i = c - nex;
+        piece = kp && kp[i] > 1 ? ~0u : 0u;  // a split row: its pieces did it
+      }
+      hdr[0] = i;
+      hdr[1] = piece;
+    }
     __syncthreads();
-    const uint32_t idx = hdr[0];
+    const uint32_t idx = hdr[0], hp = hdr[1];
     __syncthreads();
     if (idx >= nrows) break;
+    const bool skip = hp == ~0u;
+    const uint32_t piece = skip ? 0u : hp;
+    const uint32_t K = kp ? kp[idx] : 1u;
     const uint32_t r = rows[idx];
     const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1], nA = a1 - a0;
     const uint32_t left = a.left[r], hi = a.right[r];
     uint64_t total = 0;
     bool overflow = false;
-    if (nA != 0 && a.products[r] != 0 && hi >= left) {
+    if (!skip && nA != 0 && a.products[r] != 0 && hi >= left) {
       // tile width from the row density: about 3/4 of the slots per tile
       const uint64_t span = (uint64_t)hi - left + 1;
       const uint64_t prod = a.products[r];
-      uint64_t est = prod < span ? prod : span;
-      if (o.nnz_hint) est = o.nnz_hint[r];
-      uint32_t lw = logT;
-      while (lw < MAXLOG && (span >> lw) >= 1 &&
-             est * (1ull << (lw + 1)) <= (uint64_t)cap * 3 / 4 * span)
-        ++lw;
+      const uint64_t est = o.nnz_hint ? o.nnz_hint[r] : (prod < span ? prod : span);
+      const uint32_t lw = r1_tile_log(logT, MAXLOG, span, est, cap);
       const uint32_t step = lw - logT;
       const uint64_t off = o.off[r];
       // estimated capacity (estimation workflow): a row whose outputs exceed
@@ -156,7 +239,7 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
       // tiles are counted from the row's column block (block-diagonal
       // batches), else from 0
       const uint32_t base_r = bbase ? bbase[a.a_ci[a0]] : 0u;
-      const uint32_t t_begin = (left - base_r) >> lw, t_end = (hi - base_r) >> lw;
+      uint32_t t_begin = (left - base_r) >> lw, t_end = (hi - base_r) >> lw;
       // this thread's bitmap words [w0, w0 + wpt) of the tile
       const uint32_t nwords = 1u << (lw - 5);
       const uint32_t wpt = nwords >= BS ? nwords / BS : 1u;
@@ -165,7 +248,7 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
       const bool one_chunk = nA <= BS;
       const bool wide = tsw != nullptr && lw == MAXLOG;
       auto tsv = [&](uint32_t k, uint32_t tt) -> uint32_t {
-        if (wide) return tsw[(uint64_t)k * kWideStarts + min(kWideStarts - 1, tt)];
+        if (wide) return tsw[(uint64_t)k * wper + min(wper - 1, tt)];
         return ts[(uint64_t)k * T1 + min(T1 - 1, tt << step)];
       };
       // stored bitmap words of the tile at wbase (zero outside the row)
@@ -330,6 +413,24 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
         }
       };
 
+      // a piece (stored rows only): tiles [t_begin, t_end] of its share,
+      // after the row outputs counted in the stored words before it
+      if (K > 1) {
+        const uint32_t nt = t_end - t_begin + 1, t0 = t_begin;
+        t_begin = t0 + (uint32_t)((uint64_t)piece * nt / K);
+        t_end = t0 + (uint32_t)((uint64_t)(piece + 1) * nt / K) - 1;
+        if (piece) {
+          const uint32_t g4 = (uint32_t)((((uint64_t)t_begin << lw) - lo4) >> 7);
+          uint32_t c = 0;
+          for (uint32_t g = tid; g < g4; g += BS) {
+            const uint4 v = bm[bo + g];
+            c += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+          }
+          uint32_t d0, d1, d2, before, t1, t2;
+          scan3(c, 0u, 0u, d0, d1, d2, before, t1, t2);
+          total = before;
+        }
+      }
       // single-chunk rows: each thread keeps its A entry and the bounds of
       // its segment; the next tile's end is loaded one tile ahead
       uint32_t my_k = 0, ts_cur = 0, ts_end = 0;
@@ -406,18 +507,19 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
         total += m;
       }
     }
-    if (tid == 0) {
+    if (tid == 0 && !skip) {
       if (overflow) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
-      if (o.nnz) o.nnz[r] = overflow ? 0u : (uint32_t)total;
+      // the last piece's total counts the whole row
+      if (o.nnz && piece == K - 1) o.nnz[r] = overflow ? 0u : (uint32_t)total;
     }
   }
 }
 
 int launch_num_rank1(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                     const uint32_t *ts, uint32_t T1, const uint32_t *tsw,
+                     const uint32_t *ts, uint32_t T1, const uint32_t *tsw, uint32_t wper,
                      const uint32_t *bbase, const void *bm, const uint64_t *bmoff,
                      const NumOut &o, void *scratch, hipStream_t s, const RowSel &sel,
-                     uint32_t bs) {
+                     uint32_t bs, const R1Pieces &pc) {
   if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
   if (bs != 512) bs = 1024;
@@ -440,11 +542,21 @@ int launch_num_rank1(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uin
     const uint32_t cap = (uint32_t)(((budget - fixed) / 12) & ~63ull);                    \
     const size_t lds = fixed + (size_t)cap * 12;                                          \
     const uint32_t full = std::max<uint32_t>(1, t.num_cu) * R1Shape<B>::PER_CU;           \
-    const uint32_t grid = sel.bounds ? full : std::min<uint32_t>(nrows, full);            \
+    uint32_t grid = sel.bounds ? full : std::min<uint32_t>(nrows, full);                  \
+    const bool split = pc.kp && !sel.bounds && bmoff && !bbase && !o.cap;                 \
+    if (split) {                                                                          \
+      OC_TRY(zero_u32(pc.n_extra, 1, s));                                                 \
+      hipLaunchKernelGGL(k_r1_pieces<B>, dim3(cdiv(nrows, 256)), dim3(256), 0, s, rows,   \
+                         nrows, a.products, a.left, a.right, o.nnz_hint, bmoff, logT,     \
+                         cap, pc.per_piece, pc.kp, pc.extra, pc.n_extra, pc.max_extra);   \
+      LAUNCH_CHECK();                                                                     \
+      grid = full;                                                                        \
+    }                                                                                     \
     OC_TRY(set_lds(k_num_rank1<T, B>, lds));                                              \
     hipLaunchKernelGGL((k_num_rank1<T, B>), dim3(grid), dim3(B), lds, s, kargs<T>(a),     \
-                       rows, nrows, counter, logT, ts, T1, tsw, bbase,                    \
-                       (const uint4 *)bm, bmoff, cap, kout<T>(o), sel.bounds, sel.err);   \
+                       rows, nrows, counter, logT, ts, T1, tsw, wper, bbase,              \
+                       (const uint4 *)bm, bmoff, cap, kout<T>(o), sel.bounds, sel.err,    \
+                       split ? pc.kp : nullptr, pc.extra, pc.n_extra);                    \
   } while (0)
   if (a.A.vtype == OC_F64) {
     if (bs == 512) GO(double, 512);

@@ -175,10 +175,12 @@ int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
                      const uint64_t *bmoff = nullptr, void *bm = nullptr,
                      const RowSel &sel = RowSel{});
 // Windowed kernels take the bin row list (device) and its host-known
-// length; cursor scratch is grid * max_a_nnz u32.
+// length; cursor scratch is grid * max_a_nnz u32.  bmoff/bm as for
+// launch_sym_dense: rows with a stored bitmap get their windows' words.
 int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                       uint32_t max_a_nnz, uint32_t *nnz, void *scratch,
-                      hipStream_t s, const RowSel &sel = RowSel{});
+                      hipStream_t s, const RowSel &sel = RowSel{},
+                      const uint64_t *bmoff = nullptr, void *bm = nullptr);
 size_t window_scratch_bytes(uint32_t max_a_nnz);
 
 int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
@@ -202,8 +204,9 @@ int launch_num_tiles(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uin
                      uint32_t bs, const uint32_t *ts, uint32_t T1, const NumOut &o,
                      void *scratch, hipStream_t s, const uint32_t *nrows_dev = nullptr);
 // Rank-compressed tiles (tile-start table of granularity 2^logT columns;
-// tsw, when given, the compact table of the widest tile: wide_starts_plan
-// entries per B row, built by launch_tile_starts at rank_wide_log(bs)).
+// tsw, when given, the compact table of the widest tile: wper =
+// wide_starts_plan entries per B row (its tiles + 1), built by
+// launch_tile_starts at rank_wide_log(bs)).
 uint32_t rank_wide_log(uint32_t bs);
 int wide_starts_plan(uint64_t b_cols, uint32_t bs, size_t *entries_per_row);
 // the same for a table of 2^lw-column tiles
@@ -211,19 +214,30 @@ int wide_starts_for_log(uint64_t b_cols, uint32_t lw, size_t *entries_per_row);
 // bbase (optional, per B row): the first column of the row's block in a
 // block-diagonal product; the tile tables are then built from the same bases.
 int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, const uint32_t *bbase,
+                    const uint32_t *ts, uint32_t T1, const uint32_t *tsw, uint32_t wper,
+                    const uint32_t *bbase,
                     uint32_t pcap, uint32_t bs, const NumOut &o, void *scratch, hipStream_t s,
                     const RowSel &sel = RowSel{});
 // One-gather rank tiles (k_rank.hip), 1024 or 512 threads (bs), tiles up
 // to 2^rank1_wide_log(bs) columns; tsw only when built at that width.  Rows
 // with a stored bitmap (bmoff[r] != ~0, no block bases) gather each
 // product once; the others mark their tiles first.
+// pc (optional): rows with a stored bitmap and more than pc.per_piece
+// products are split into pieces of consecutive tiles taken by different
+// workgroups (eager runs of the precise workflow): kp holds nrows bytes,
+// extra 2 * max_extra u32, n_extra one u32.
+struct R1Pieces {
+  uint8_t *kp = nullptr;
+  uint32_t *extra = nullptr, *n_extra = nullptr;
+  uint32_t max_extra = 0, per_piece = 0;
+};
 uint32_t rank1_wide_log(uint32_t bs);
 int launch_num_rank1(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint32_t logT,
-                     const uint32_t *ts, uint32_t T1, const uint32_t *tsw,
+                     const uint32_t *ts, uint32_t T1, const uint32_t *tsw, uint32_t wper,
                      const uint32_t *bbase, const void *bm, const uint64_t *bmoff,
                      const NumOut &o, void *scratch, hipStream_t s,
-                     const RowSel &sel = RowSel{}, uint32_t bs = 1024);
+                     const RowSel &sel = RowSel{}, uint32_t bs = 1024,
+                     const R1Pieces &pc = R1Pieces{});
 // Stored-bitmap offsets (uint4 units) for the listed rows with at least
 // min_products products; bmoff[0..m) = ~0 elsewhere; offs[n] = total.
 // sz: n u32, offs: n + 1 u64, scan_tmp: scan_temp_bytes(n).

@@ -252,6 +252,8 @@ struct oc_spgemm {
     uint32_t tile_logw = 0, tile_t1 = 0, max_a_nnz = 0;
     int tiles_ok = -1;
     void *win_scratch = nullptr;
+    uint8_t *pc_kp = nullptr;  // piece lists of the long-row kernel
+    uint32_t *pc_extra = nullptr, *pc_next = nullptr, pc_cap = 0;
     uint32_t band_n = 0, band_nbands = 0, band_logw = 0;
     uint32_t *band_rows = nullptr, *band_ts = nullptr, *band_counter = nullptr;
     uint64_t *band_pref = nullptr;
@@ -385,6 +387,9 @@ struct Run {
   uint8_t *hll_b = nullptr;
   uint32_t *learned_est = nullptr;  // learned capacities (learned_capacities)
   void *win_scratch = nullptr;
+  // piece lists of the long-row kernel: kp for pc_cap list entries
+  uint8_t *pc_kp = nullptr;
+  uint32_t *pc_extra = nullptr, *pc_next = nullptr, pc_cap = 0;
   uint32_t *tile_ts = nullptr, *tile_tsw = nullptr;
   const uint32_t *block_base = nullptr;  // per-B-row block bases of the tables
   uint32_t tile_logw = 0, tile_t1 = 0;
@@ -474,22 +479,23 @@ struct Run {
     return OC_OK;
   }
 
-  // Row bitmaps of the dense symbolic bins (rows with at least
-  // kBitmapMinProducts products), stored by the symbolic pass for the
-  // one-gather rank tiles of the numeric pass.  Skipped for block-diagonal
+  // Row bitmaps of the windowed and dense symbolic bins (rows with at
+  // least kBitmapMinProducts products), stored by the symbolic pass for the
+  // one-gather rank tiles of the numeric pass.  The bins' rows are
+  // [hoff[6], hoff[14]) (bin 7 is never used).  Skipped for block-diagonal
   // batches and when the bitmaps would take more than a fifth of the device.
   int row_bitmaps(const uint32_t *brows, const uint32_t *hoff) {
     constexpr uint32_t kBitmapMinProducts = 2048;
     h->bitmap_bytes = 0;
     const bool blocks = h->b_block_base != nullptr && h->b_block_rp == (const void *)B.rp;
-    if (h->cfg.use_tiled_numeric != 3 || blocks || hoff[14] <= hoff[8]) return OC_OK;
-    const uint32_t nd = hoff[14] - hoff[8];
+    if (h->cfg.use_tiled_numeric != 3 || blocks || hoff[14] <= hoff[6]) return OC_OK;
+    const uint32_t nd = hoff[14] - hoff[6];
     uint64_t *bmoff, *offs;
     uint32_t *sz;
     OC_TRY(ar.get(&bmoff, A.rows));
     OC_TRY(ar.get(&sz, nd));
     OC_TRY(ar.get(&offs, (uint64_t)nd + 1));
-    OC_TRY(launch_bitmap_offsets(brows + hoff[8], nd, A.rows, ri.products, ri.left, ri.right,
+    OC_TRY(launch_bitmap_offsets(brows + hoff[6], nd, A.rows, ri.products, ri.left, ri.right,
                                  kBitmapMinProducts, sz, offs, scan_tmp, bmoff, s));
     uint64_t n4 = 0;
     OC_TRY(readback(&n4, offs + nd, 1, s));
@@ -511,12 +517,33 @@ struct Run {
     OC_TRY(tiles());
     if (tiles_ok && h->cfg.use_tiled_numeric == 3) {
       const uint32_t bs = h->cfg.tile_block == 512 ? 512u : 1024u;
+      // rows of more than kPieceProducts products (stored bitmaps, eager
+      // precise runs) are split over several workgroups
+      constexpr uint32_t kPieceProducts = 1u << 21, kMaxPieces = 1u << 16;
+      R1Pieces pc;
+      if (row_bm && !sel.bounds && !o.cap && n) {
+        if (n > pc_cap && !replay) {  // a replay reuses the plan's lists
+          OC_TRY(ar.get(&pc_kp, n));
+          if (!pc_extra) {
+            OC_TRY(ar.get(&pc_extra, 2 * kMaxPieces));
+            OC_TRY(ar.get(&pc_next, 1));
+          }
+          pc_cap = n;
+        }
+        if (n <= pc_cap) {
+          pc.kp = pc_kp;
+          pc.extra = pc_extra;
+          pc.n_extra = pc_next;
+          pc.max_extra = kMaxPieces;
+          pc.per_piece = kPieceProducts;
+        }
+      }
       return launch_num_rank1(acc(), rows, n, tile_logw, tile_ts, tile_t1,
-                              tsw_log == rank1_wide_log(bs) ? tile_tsw : nullptr,
-                              block_base, row_bm, row_bmoff, o, win_scratch, s, sel, bs);
+                              tsw_log == rank1_wide_log(bs) ? tile_tsw : nullptr, tsw_per,
+                              block_base, row_bm, row_bmoff, o, win_scratch, s, sel, bs, pc);
     }
     if (tiles_ok && h->cfg.use_tiled_numeric == 2)
-      return launch_num_rank(acc(), rows, n, tile_logw, tile_ts, tile_t1, tile_tsw,
+      return launch_num_rank(acc(), rows, n, tile_logw, tile_ts, tile_t1, tile_tsw, tsw_per,
                              block_base, (uint32_t)h->cfg.tile_products_cap,
                              (uint32_t)h->cfg.tile_block, o,
                              win_scratch, s, sel);
@@ -1048,7 +1075,7 @@ int Run::run_precise() {
   if (hoff[7] > hoff[6]) {
     OC_TRY(window_scratch());
     OC_TRY(launch_sym_window(a, brows + hoff[6], hoff[7] - hoff[6], h->max_a_nnz, nnz,
-                             win_scratch, s));
+                             win_scratch, s, RowSel{}, row_bmoff, row_bm));
   }
   fam("sym_window", "symbolic_window", false, brows + hoff[6], hoff[7] - hoff[6]);
   mk.mark("symbolic_window");
@@ -1150,6 +1177,10 @@ int Run::run_precise() {
     p.tile_t1 = tile_t1;
     p.tiles_ok = tiles_ok;
     p.win_scratch = win_scratch;
+    p.pc_kp = pc_kp;
+    p.pc_extra = pc_extra;
+    p.pc_next = pc_next;
+    p.pc_cap = pc_cap;
     p.max_a_nnz = h->max_a_nnz;
     p.band_n = band.n;
     p.band_nbands = band.nbands;
@@ -1766,6 +1797,10 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   r.tile_t1 = p.tile_t1;
   r.tiles_ok = p.tiles_ok;
   r.win_scratch = p.win_scratch;
+  r.pc_kp = p.pc_kp;
+  r.pc_extra = p.pc_extra;
+  r.pc_next = p.pc_next;
+  r.pc_cap = p.pc_cap;
   r.band.n = p.band_n;
   r.band.nbands = p.band_nbands;
   r.band.logw = p.band_logw;

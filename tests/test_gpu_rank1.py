@@ -145,3 +145,51 @@ def test_rank1_plan_replay(gpu):
         sp.replay()
         worst, _ = check_product(A, A, dC.download(), TOL64, f"rank1 replay {it}")
         log(f"rank1 replay {it}: err {worst:.2e}")
+
+
+def heavy_rows(gpu, log_cols, seed=9):
+    """B of 2^15 rows over 2^log_cols columns, 64-255 entries each; A rows
+    of 32,000, 20,000, 700 and 3 entries: the first two have about 5M and
+    3M products (more than the 2^21 of one piece) and spread over all
+    columns."""
+    rng = np.random.default_rng(seed)
+    nb, n = 1 << 15, 1 << log_cols
+    blen = rng.integers(64, 256, nb)
+    bcols = [np.sort(rng.choice(n, int(L), replace=False)) for L in blen]
+    brp = np.zeros(nb + 1, np.uint32)
+    np.cumsum(blen, out=brp[1:])
+    B = gpu.CSR(nb, n, brp, np.concatenate(bcols).astype(np.uint32),
+                rng.uniform(-1, 1, int(brp[-1])))
+    alen = np.array([32000, 20000, 700, 3])
+    acols = [np.sort(rng.choice(nb, int(L), replace=False)) for L in alen]
+    arp = np.zeros(len(alen) + 1, np.uint32)
+    np.cumsum(alen, out=arp[1:])
+    A = gpu.CSR(len(alen), nb, arp, np.concatenate(acols).astype(np.uint32),
+                rng.uniform(-1, 1, int(arp[-1])))
+    return A, B
+
+
+@pytest.mark.parametrize("log_cols", [20, 21])
+def test_rank1_row_pieces(gpu, log_cols):
+    """Rows of more than 2^21 products with stored bitmaps are split into
+    pieces of consecutive tiles on different workgroups, each starting from
+    the outputs counted in the stored words before it.  Over 2^20 columns
+    the bitmaps come from the dense symbolic kernel; over 2^21 (wider than
+    its LDS bitmap) from the windowed symbolic kernel.  Precise workflow,
+    then a value-only replay of the plan."""
+    log(f"case: rank1 row pieces, B 2^15 x 2^{log_cols}")
+    A, B = heavy_rows(gpu, log_cols)
+    C, st = _run(gpu, A, B, gpu.Config(use_tiled_numeric=3), 0)
+    assert st["decision"]["stored_row_bitmap_bytes"] > 0
+    worst, _ = check_product(A, B, C, TOL64, f"row pieces 2^{log_cols}")
+    log(f"row pieces 2^{log_cols}: nnz {C.nnz}, err {worst:.2e}")
+    dA, dB = A.upload(), B.upload()
+    sp = gpu.SpGEMM(gpu.Config(use_tiled_numeric=3))
+    sp.set_workflow(0)
+    dC = sp.plan(dA, dB)
+    vals = np.ascontiguousarray(np.random.default_rng(3).uniform(-1, 1, A.nnz))
+    gpu._check(gpu.lib.oc_memcpy_htod(dA.m.values, vals.ctypes.data, vals.nbytes))
+    A.values = vals
+    sp.replay()
+    worst, _ = check_product(A, B, dC.download(), TOL64, f"row pieces replay 2^{log_cols}")
+    log(f"row pieces replay 2^{log_cols}: err {worst:.2e}")
