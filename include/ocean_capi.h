@@ -50,7 +50,7 @@ typedef struct oc_csr {
   uint32_t *col_ids;  /* nnz entries, ascending within a row           */
   void *values;       /* nnz entries of vtype                          */
   int32_t on_device;  /* 1: HIP device memory, 0: host memory          */
-  int32_t owned;      /* 1: allocated by this library (oc_csr_free)    */
+  int32_t owned;      /* nonzero: allocated by this library (oc_csr_free) */
 } oc_csr;
 
 /* Allocate a host or device matrix (owned).

@@ -15,11 +15,101 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "oc_internal.hpp"
 
 namespace oc {
+
+namespace {
+struct OutBlock {
+  int dev;
+  size_t size;
+};
+struct OutCache {
+  std::mutex mu;
+  std::unordered_map<void *, OutBlock> blocks;       // every block handed out or cached
+  std::map<int, std::multimap<size_t, void *>> idle;  // per device: cached blocks by size
+  std::map<int, size_t> idle_bytes;
+};
+OutCache &out_cache() {
+  static OutCache *c = new OutCache();  // never destroyed: freed at process exit
+  return *c;
+}
+// Gives the idle blocks of device dev back to the driver, smallest first
+// (large blocks serve any request), until at most keep bytes stay cached
+// (lock held).
+void out_shrink(OutCache &c, int dev, size_t keep) {
+  auto &idle = c.idle[dev];
+  while (c.idle_bytes[dev] > keep && !idle.empty()) {
+    auto it = idle.begin();
+    (void)hipFree(it->second);
+    c.idle_bytes[dev] -= it->first;
+    c.blocks.erase(it->second);
+    idle.erase(it);
+  }
+}
+}  // namespace
+
+int out_alloc(void **p, size_t bytes) {
+  const size_t g = ((std::max<size_t>(bytes, 1) + (2u << 20) - 1) >> 21) << 21;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("out_alloc: no current device");
+    return OC_ERR_HIP;
+  }
+  OutCache &c = out_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto &idle = c.idle[dev];
+  // the smallest idle block that holds the request; requests under 64 MiB
+  // take blocks of at most four times their size
+  auto it = idle.lower_bound(g);
+  if (it != idle.end() && (g >= (64u << 20) || it->first <= 4 * g)) {
+    *p = it->second;
+    c.idle_bytes[dev] -= it->first;
+    idle.erase(it);
+    return OC_OK;
+  }
+  hipError_t e = hipMalloc(p, g);
+  if (e == hipErrorOutOfMemory) {  // give the cached blocks back and retry
+    (void)hipGetLastError();
+    out_shrink(c, dev, 0);
+    e = hipMalloc(p, g);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    set_error("device allocation of %zu bytes for a product output failed: %s", g,
+              hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? OC_ERR_NOMEM : OC_ERR_HIP;
+  }
+  c.blocks[*p] = OutBlock{dev, g};
+  return OC_OK;
+}
+
+void out_free(void *p) {
+  if (!p) return;
+  (void)hipDeviceSynchronize();  // as hipFree: no work may still use it
+  OutCache &c = out_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto b = c.blocks.find(p);
+  if (b == c.blocks.end()) {  // not from the cache
+    (void)hipFree(p);
+    return;
+  }
+  const int dev = b->second.dev;
+  c.idle[dev].emplace(b->second.size, p);
+  c.idle_bytes[dev] += b->second.size;
+  size_t fr = 0, tot = 0;
+  int cur = 0;
+  if (hipGetDevice(&cur) == hipSuccess && cur == dev && hipMemGetInfo(&fr, &tot) == hipSuccess)
+    out_shrink(c, dev, tot / 4);
+  (void)hipGetLastError();
+}
 
 static thread_local std::string g_err;
 
@@ -88,7 +178,11 @@ extern "C" int oc_csr_alloc(oc_csr *m, uint64_t rows, uint64_t cols,
 
 extern "C" void oc_csr_free(oc_csr *m) {
   if (!m) return;
-  if (m->owned == 2) {  // stream-ordered pool allocation (SpGEMM output)
+  if (m->owned == 3) {  // product output from the output cache
+    out_free(m->row_offsets);
+    out_free(m->col_ids);
+    out_free(m->values);
+  } else if (m->owned == 2) {  // stream-ordered pool allocation
     if (m->row_offsets) (void)hipFreeAsync(m->row_offsets, nullptr);
     if (m->col_ids) (void)hipFreeAsync(m->col_ids, nullptr);
     if (m->values) (void)hipFreeAsync(m->values, nullptr);

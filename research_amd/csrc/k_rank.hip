@@ -95,13 +95,14 @@ __device__ __forceinline__ uint32_t r1_tile_log(uint32_t logT, uint32_t maxlog, 
   return lw;
 }
 
-// Pieces of long rows.  A row with a stored bitmap and more than per_piece
-// products is cut into K pieces of consecutive tiles (K at most its tiles
-// and kR1MaxPieces), which different workgroups take: a piece finds the
-// outputs of the row before its first tile from the stored bitmap, so no
-// piece waits for another.  kp[i] = K of list entry i; all K pieces are
-// appended to `extra` as (i, piece) pairs, at most max_extra of them (a row
-// that does not fit stays whole).  The kernel takes the pieces before the
+// Pieces of long rows.  A row with more than per_piece products is cut into
+// K pieces of consecutive tiles (K at most its tiles and kR1MaxPieces),
+// which different workgroups take.  A piece finds the outputs of the row
+// before its first tile from the stored bitmap, or, for a row without one
+// (only when `counted`), from the per-piece counts of a count-only pass of
+// the kernel; no piece waits for another.  kp[i] = K of list entry i; all K
+// pieces are appended to `extra` as (i, piece) pairs, consecutive, at most
+// max_extra of them (a row that does not fit stays whole).  The kernel takes the pieces before the
 // list, so the heaviest work starts first (taken after the list, the
 // pieces left a tail: R-MAT 2^20 went from 110.7 to 114.3 ms).
 constexpr uint32_t kR1MaxPieces = 64;
@@ -112,14 +113,15 @@ __global__ __launch_bounds__(256) void k_r1_pieces(
     const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
     const uint32_t *__restrict__ nnz_hint, const uint64_t *__restrict__ bmoff, uint32_t logT,
     uint32_t cap, uint32_t per_piece, uint8_t *__restrict__ kp, uint32_t *__restrict__ extra,
-    uint32_t *n_extra, uint32_t max_extra) {
+    uint32_t *n_extra, uint32_t max_extra, uint32_t counted) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nrows) return;
   const uint32_t r = rows[i];
   const uint32_t l = left[r], h = right[r];
   const uint64_t prod = products[r];
   uint32_t K = 1;
-  if (bmoff[r] != ~0ull && h >= l && prod > per_piece) {
+  const bool stored = bmoff && bmoff[r] != ~0ull;
+  if ((stored || counted) && h >= l && prod > per_piece) {
     const uint64_t span = (uint64_t)h - l + 1;
     const uint64_t est = nnz_hint ? nnz_hint[r] : (prod < span ? prod : span);
     const uint32_t lw = r1_tile_log(logT, R1Shape<BS>::MAXLOG, span, est, cap);
@@ -154,7 +156,9 @@ __global__ __launch_bounds__(256) void k_r1_pieces(
 
 // One 1024-thread workgroup per row or row piece, taken from an atomic
 // counter: the pieces of k_r1_pieces first (when kp is given), then list
-// entries [0, nrows), skipping those of split rows.  Requires fewer than 2^31
+// entries [0, nrows), skipping those of split rows.  count_only: only the
+// pieces of rows without a stored bitmap, whose outputs are counted into
+// pcnt (indexed like `extra`) and nothing is written.  Requires fewer than 2^31
 // products per row (the analysis stores them as u32).  bm/bmoff: stored
 // row bitmaps (uint4 groups from column left & ~127; bmoff[r] = ~0 for rows
 // without one); only used without block bases.
@@ -166,7 +170,7 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
     const uint4 *__restrict__ bm, const uint64_t *__restrict__ bmoff, uint32_t cap,
     KOut<T> o, const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err,
     const uint8_t *__restrict__ kp, const uint32_t *__restrict__ extra,
-    const uint32_t *__restrict__ n_extra) {
+    const uint32_t *__restrict__ n_extra, uint32_t *__restrict__ pcnt, uint32_t count_only) {
   if (err && *err) return;  // a capacity check of the run failed
   if (bounds) {             // rows [bounds[0], bounds[1]) of the list
     rows += bounds[0];
@@ -198,19 +202,39 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
   for (;;) {
     if (tid == 0) {
       const uint32_t c = atomicAdd(counter, 1u);
-      uint32_t i = ~0u, piece = 0;
+      // piece ~0: skip the entry; counted pieces: the outputs before it and
+      // the row's total from the count pass
+      uint32_t i = ~0u, piece = 0, before = 0, rtot = 0, counted = 0;
       if (c < nex) {
         i = extra[2 * c];
         piece = extra[2 * c + 1];
-      } else if (c - nex < nrows) {
+        const bool st = bbase == nullptr && bmoff && bmoff[rows[i]] != ~0ull;
+        if (count_only) {
+          if (st) piece = ~0u;  // a stored bitmap needs no count
+        } else if (!st) {
+          counted = 1;
+          const uint32_t first = c - piece, K = kp[i];
+          for (uint32_t q = 0; q < K; ++q) {
+            const uint32_t v = pcnt[first + q];
+            before += q < piece ? v : 0u;
+            rtot += v;
+          }
+        }
+      } else if (!count_only && c - nex < nrows) {
         i = c - nex;
         piece = kp && kp[i] > 1 ? ~0u : 0u;  // a split row: its pieces did it
       }
       hdr[0] = i;
       hdr[1] = piece;
+      hdr[2] = before;
+      hdr[3] = rtot;
+      hdr[4] = counted;
+      hdr[5] = c;
     }
     __syncthreads();
-    const uint32_t idx = hdr[0], hp = hdr[1];
+    const uint32_t idx = hdr[0], hp = hdr[1], h_before = hdr[2], h_rtot = hdr[3];
+    const bool counted = hdr[4] != 0u;
+    const uint32_t claim = hdr[5];
     __syncthreads();
     if (idx >= nrows) break;
     const bool skip = hp == ~0u;
@@ -219,9 +243,12 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
     const uint32_t r = rows[idx];
     const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1], nA = a1 - a0;
     const uint32_t left = a.left[r], hi = a.right[r];
-    uint64_t total = 0;
+    uint64_t total = counted ? h_before : 0u;
+    // a counted row over its estimated capacity: no piece works on it, and
+    // piece 0 lists it for the recount
+    const bool row_over = counted && o.cap && h_rtot > o.cap[r];
     bool overflow = false;
-    if (!skip && nA != 0 && a.products[r] != 0 && hi >= left) {
+    if (!skip && !row_over && nA != 0 && a.products[r] != 0 && hi >= left) {
       // tile width from the row density: about 3/4 of the slots per tile
       const uint64_t span = (uint64_t)hi - left + 1;
       const uint64_t prod = a.products[r];
@@ -413,13 +440,13 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
         }
       };
 
-      // a piece (stored rows only): tiles [t_begin, t_end] of its share,
-      // after the row outputs counted in the stored words before it
+      // a piece: tiles [t_begin, t_end] of its share, after the row outputs
+      // counted in the stored words before it (or by the count pass)
       if (K > 1) {
         const uint32_t nt = t_end - t_begin + 1, t0 = t_begin;
         t_begin = t0 + (uint32_t)((uint64_t)piece * nt / K);
         t_end = t0 + (uint32_t)((uint64_t)(piece + 1) * nt / K) - 1;
-        if (piece) {
+        if (piece && stored) {
           const uint32_t g4 = (uint32_t)((((uint64_t)t_begin << lw) - lo4) >> 7);
           uint32_t c = 0;
           for (uint32_t g = tid; g < g4; g += BS) {
@@ -483,12 +510,13 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
           }
           m = prefix_marks();
         }
-        if (total + m > rcap) {  // m and total are uniform over the block
+        // (the count pass only counts: no capacity check, sums or writes)
+        if (!count_only && total + m > rcap) {  // m and total are uniform over the block
           clear_words();
           overflow = true;
           break;
         }
-        for (uint32_t r0 = 0; r0 < m; r0 += cap) {
+        for (uint32_t r0 = 0; !count_only && r0 < m; r0 += cap) {
           const uint32_t r1 = min(m, r0 + cap);
           if (r0) __syncthreads();  // the previous range's slots are cleared
           if (one_chunk) {
@@ -508,9 +536,13 @@ __global__ __launch_bounds__(BS) void k_num_rank1(
       }
     }
     if (tid == 0 && !skip) {
-      if (overflow) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
-      // the last piece's total counts the whole row
-      if (o.nnz && piece == K - 1) o.nnz[r] = overflow ? 0u : (uint32_t)total;
+      if (count_only) {
+        pcnt[claim] = (uint32_t)total;
+      } else {
+        if (overflow || (row_over && piece == 0)) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
+        // the last piece's total counts the whole row
+        if (o.nnz && piece == K - 1) o.nnz[r] = overflow || row_over ? 0u : (uint32_t)total;
+      }
     }
   }
 }
@@ -543,20 +575,29 @@ int launch_num_rank1(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uin
     const size_t lds = fixed + (size_t)cap * 12;                                          \
     const uint32_t full = std::max<uint32_t>(1, t.num_cu) * R1Shape<B>::PER_CU;           \
     uint32_t grid = sel.bounds ? full : std::min<uint32_t>(nrows, full);                  \
-    const bool split = pc.kp && !sel.bounds && bmoff && !bbase && !o.cap;                 \
+    OC_TRY(set_lds(k_num_rank1<T, B>, lds));                                              \
+    const bool split = pc.kp && !sel.bounds && !bbase && !(o.cap && bmoff);               \
     if (split) {                                                                          \
       OC_TRY(zero_u32(pc.n_extra, 1, s));                                                 \
       hipLaunchKernelGGL(k_r1_pieces<B>, dim3(cdiv(nrows, 256)), dim3(256), 0, s, rows,   \
                          nrows, a.products, a.left, a.right, o.nnz_hint, bmoff, logT,     \
-                         cap, pc.per_piece, pc.kp, pc.extra, pc.n_extra, pc.max_extra);   \
+                         cap, pc.per_piece, pc.kp, pc.extra, pc.n_extra, pc.max_extra,    \
+                         pc.pcnt ? 1u : 0u);                                              \
       LAUNCH_CHECK();                                                                     \
       grid = full;                                                                        \
+      if (pc.pcnt) { /* count the pieces of rows without a stored bitmap */               \
+        hipLaunchKernelGGL((k_num_rank1<T, B>), dim3(grid), dim3(B), lds, s, kargs<T>(a), \
+                           rows, nrows, counter, logT, ts, T1, tsw, wper, bbase,          \
+                           (const uint4 *)bm, bmoff, cap, kout<T>(o), sel.bounds,         \
+                           sel.err, pc.kp, pc.extra, pc.n_extra, pc.pcnt, 1u);            \
+        LAUNCH_CHECK();                                                                   \
+        OC_TRY(zero_u32(counter, 1, s));                                                  \
+      }                                                                                   \
     }                                                                                     \
-    OC_TRY(set_lds(k_num_rank1<T, B>, lds));                                              \
     hipLaunchKernelGGL((k_num_rank1<T, B>), dim3(grid), dim3(B), lds, s, kargs<T>(a),     \
                        rows, nrows, counter, logT, ts, T1, tsw, wper, bbase,              \
                        (const uint4 *)bm, bmoff, cap, kout<T>(o), sel.bounds, sel.err,    \
-                       split ? pc.kp : nullptr, pc.extra, pc.n_extra);                    \
+                       split ? pc.kp : nullptr, pc.extra, pc.n_extra, pc.pcnt, 0u);       \
   } while (0)
   if (a.A.vtype == OC_F64) {
     if (bs == 512) GO(double, 512);

@@ -222,13 +222,14 @@ int launch_num_rank(const AccArgs &a, const uint32_t *rows, uint32_t nrows, uint
 // to 2^rank1_wide_log(bs) columns; tsw only when built at that width.  Rows
 // with a stored bitmap (bmoff[r] != ~0, no block bases) gather each
 // product once; the others mark their tiles first.
-// pc (optional): rows with a stored bitmap and more than pc.per_piece
-// products are split into pieces of consecutive tiles taken by different
-// workgroups (eager runs of the precise workflow): kp holds nrows bytes,
-// extra 2 * max_extra u32, n_extra one u32.
+// pc (optional): rows with more than pc.per_piece products are split into
+// pieces of consecutive tiles taken by different workgroups (eager runs
+// without block bases): kp holds nrows bytes, extra 2 * max_extra u32,
+// n_extra one u32.  Rows without a stored bitmap are split only with pcnt
+// (max_extra u32), which a count pass fills with each piece's outputs.
 struct R1Pieces {
   uint8_t *kp = nullptr;
-  uint32_t *extra = nullptr, *n_extra = nullptr;
+  uint32_t *extra = nullptr, *n_extra = nullptr, *pcnt = nullptr;
   uint32_t max_extra = 0, per_piece = 0;
 };
 uint32_t rank1_wide_log(uint32_t bs);

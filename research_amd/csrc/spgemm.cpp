@@ -64,6 +64,19 @@ struct ChunkStore {
     for (Chunk &c : chunks) (void)hipFree(c.base);
     chunks.clear();
   }
+  // Frees the chunks a run left untouched (call once its stream has
+  // drained, before the run's marks are reset), so the store holds what the
+  // latest run needed rather than the union of every run's sizes.  Freed
+  // memory is cleared by the driver before its next use, so the runs call
+  // this only when the store holds more than a quarter of the device.
+  void trim() {
+    size_t k = 0;
+    for (Chunk &c : chunks) {
+      if (c.used) chunks[k++] = c;
+      else (void)hipFree(c.base);
+    }
+    chunks.resize(k);
+  }
 };
 
 // One run's allocations from the handle's chunk store.
@@ -79,14 +92,22 @@ struct Arena {
   template <typename T>
   int get(T **p, uint64_t count) {
     const size_t bytes = ((size_t)std::max<uint64_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+    // best fit: the chunk with the least room that holds the request, so
+    // small requests do not keep large chunks in use
+    size_t best = cs->chunks.size();
     for (size_t i = 0; i < cs->chunks.size(); ++i) {
-      ChunkStore::Chunk &c = cs->chunks[i];
-      if (c.size - c.used >= bytes) {
-        *p = (T *)(c.base + c.used);
-        recs.push_back(Rec{i, c.used, (void *)*p});
-        c.used += bytes;
-        return OC_OK;
-      }
+      const ChunkStore::Chunk &c = cs->chunks[i];
+      if (c.size - c.used >= bytes &&
+          (best == cs->chunks.size() ||
+           c.size - c.used < cs->chunks[best].size - cs->chunks[best].used))
+        best = i;
+    }
+    if (best < cs->chunks.size()) {
+      ChunkStore::Chunk &c = cs->chunks[best];
+      *p = (T *)(c.base + c.used);
+      recs.push_back(Rec{best, c.used, (void *)*p});
+      c.used += bytes;
+      return OC_OK;
     }
     uint8_t *base = nullptr;
     const size_t size = std::max<size_t>(bytes, 64u << 20);
@@ -253,7 +274,7 @@ struct oc_spgemm {
     int tiles_ok = -1;
     void *win_scratch = nullptr;
     uint8_t *pc_kp = nullptr;  // piece lists of the long-row kernel
-    uint32_t *pc_extra = nullptr, *pc_next = nullptr, pc_cap = 0;
+    uint32_t *pc_extra = nullptr, *pc_cnt = nullptr, *pc_next = nullptr, pc_cap = 0;
     uint32_t band_n = 0, band_nbands = 0, band_logw = 0;
     uint32_t *band_rows = nullptr, *band_ts = nullptr, *band_counter = nullptr;
     uint64_t *band_pref = nullptr;
@@ -389,7 +410,7 @@ struct Run {
   void *win_scratch = nullptr;
   // piece lists of the long-row kernel: kp for pc_cap list entries
   uint8_t *pc_kp = nullptr;
-  uint32_t *pc_extra = nullptr, *pc_next = nullptr, pc_cap = 0;
+  uint32_t *pc_extra = nullptr, *pc_cnt = nullptr, *pc_next = nullptr, pc_cap = 0;
   uint32_t *tile_ts = nullptr, *tile_tsw = nullptr;
   const uint32_t *block_base = nullptr;  // per-B-row block bases of the tables
   uint32_t tile_logw = 0, tile_t1 = 0;
@@ -517,15 +538,16 @@ struct Run {
     OC_TRY(tiles());
     if (tiles_ok && h->cfg.use_tiled_numeric == 3) {
       const uint32_t bs = h->cfg.tile_block == 512 ? 512u : 1024u;
-      // rows of more than kPieceProducts products (stored bitmaps, eager
-      // precise runs) are split over several workgroups
+      // rows of more than kPieceProducts products are split over several
+      // workgroups (eager runs without block bases)
       constexpr uint32_t kPieceProducts = 1u << 21, kMaxPieces = 1u << 16;
       R1Pieces pc;
-      if (row_bm && !sel.bounds && !o.cap && n) {
+      if (!block_base && !sel.bounds && n) {
         if (n > pc_cap && !replay) {  // a replay reuses the plan's lists
           OC_TRY(ar.get(&pc_kp, n));
           if (!pc_extra) {
             OC_TRY(ar.get(&pc_extra, 2 * kMaxPieces));
+            OC_TRY(ar.get(&pc_cnt, kMaxPieces));
             OC_TRY(ar.get(&pc_next, 1));
           }
           pc_cap = n;
@@ -534,6 +556,7 @@ struct Run {
           pc.kp = pc_kp;
           pc.extra = pc_extra;
           pc.n_extra = pc_next;
+          pc.pcnt = pc_cnt;
           pc.max_extra = kMaxPieces;
           pc.per_piece = kPieceProducts;
         }
@@ -568,15 +591,15 @@ struct Run {
     C->vtype = A.vtype;
     C->off_bytes = 8;
     C->on_device = 1;
-    C->owned = 2;  // memory-pool allocation
-    HIP_TRY(hipMallocAsync(&C->row_offsets, (m + 1) * 8, s));
+    C->owned = 3;  // output cache (out_alloc)
+    OC_TRY(out_alloc(&C->row_offsets, (m + 1) * 8));
     OC_TRY(launch_scan_u32(nnz, (uint64_t *)C->row_offsets, m, scan_tmp, s));
     mk.mark("epilogue_scan");
     uint64_t nnzc = 0;
     OC_TRY(readback(&nnzc, (const uint64_t *)C->row_offsets + m, 1, s));
     C->nnz = nnzc;
-    HIP_TRY(hipMallocAsync((void **)&C->col_ids, std::max<uint64_t>(nnzc, 1) * 4, s));
-    HIP_TRY(hipMallocAsync(&C->values, std::max<uint64_t>(nnzc, 1) * vsize(A.vtype), s));
+    OC_TRY(out_alloc((void **)&C->col_ids, std::max<uint64_t>(nnzc, 1) * 4));
+    OC_TRY(out_alloc(&C->values, std::max<uint64_t>(nnzc, 1) * vsize(A.vtype)));
     mk.mark("numeric_malloc");
     return OC_OK;
   }
@@ -1179,6 +1202,7 @@ int Run::run_precise() {
     p.win_scratch = win_scratch;
     p.pc_kp = pc_kp;
     p.pc_extra = pc_extra;
+    p.pc_cnt = pc_cnt;
     p.pc_next = pc_next;
     p.pc_cap = pc_cap;
     p.max_a_nnz = h->max_a_nnz;
@@ -1685,11 +1709,11 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
     dC->vtype = dA->vtype;
     dC->off_bytes = 8;
     dC->on_device = 1;
-    dC->owned = 2;
-    HIP_TRY(hipMallocAsync(&dC->row_offsets, 8, s));
+    dC->owned = 3;
+    OC_TRY(out_alloc(&dC->row_offsets, 8));
     HIP_TRY(hipMemsetAsync(dC->row_offsets, 0, 8, s));
-    HIP_TRY(hipMallocAsync((void **)&dC->col_ids, 4, s));
-    HIP_TRY(hipMallocAsync(&dC->values, 8, s));
+    OC_TRY(out_alloc((void **)&dC->col_ids, 4));
+    OC_TRY(out_alloc(&dC->values, 8));
     h->total_products = 0;
     h->d = Decision();
   } else {
@@ -1761,8 +1785,13 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
     for (const ChunkStore::Chunk &ck : h->mem.chunks) h->plan.mem.push_back(ck.base);
     h->mem.chunks.clear();
     r.ar.recs.clear();
-  } else if (h->keep_plan) {
-    h->plan.valid = false;
+  } else {
+    if (h->keep_plan) h->plan.valid = false;
+    size_t fr = 0, tot = 0;
+    if (e == hipSuccess && hipMemGetInfo(&fr, &tot) == hipSuccess && h->mem.total() > tot / 4)
+      h->mem.trim();
+    (void)hipGetLastError();
+    r.ar.recs.clear();
   }
   r.ar.release();
   if (rc != OC_OK) {
@@ -1799,6 +1828,7 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   r.win_scratch = p.win_scratch;
   r.pc_kp = p.pc_kp;
   r.pc_extra = p.pc_extra;
+  r.pc_cnt = p.pc_cnt;
   r.pc_next = p.pc_next;
   r.pc_cap = p.pc_cap;
   r.band.n = p.band_n;

@@ -146,6 +146,10 @@ def evaluate(model: str, log_n: int = 21, degree: int = 32, panel_products: floa
     learned = ocean.SpGEMM(ocean.Config(track_stage_times=1))
     learned.set_model(m, quantile)
     t_plain = t_learned = 0.0
+    # output allocation per path: device memory freed and allocated again is
+    # cleared by the driver first, which can take hundreds of ms for a
+    # panel's output whatever the path; reported apart from the totals
+    a_plain = a_learned = 0.0
     ovf = est_total = nnz_total = products = 0
     same = True
     panels = panels_by_products(A, panel_products)
@@ -155,6 +159,7 @@ def evaluate(model: str, log_n: int = 21, degree: int = 32, panel_products: floa
             log(f"[predictor] panel {i}: rows {r0}..{r1}, plain path", flush=True)
         C1 = plain.run(P, dA)
         t_plain += plain.timer_ms("total")
+        a_plain += max(0.0, plain.timer_ms("numeric_malloc"))
         h1, o1 = C1.row_set_hash(), C1.row_offsets()
         st1 = plain.get_stats()
         C1.free()
@@ -162,6 +167,7 @@ def evaluate(model: str, log_n: int = 21, degree: int = 32, panel_products: floa
             log(f"[predictor] panel {i}: learned path", flush=True)
         C2 = learned.run(P, dA)
         t_learned += learned.timer_ms("total")
+        a_learned += max(0.0, learned.timer_ms("numeric_malloc"))
         st2 = learned.get_stats()
         same = same and np.array_equal(h1, C2.row_set_hash()) and np.array_equal(o1, C2.row_offsets())
         ovf += st2.get("estimation", {}).get("overflow_rows", 0)
@@ -177,6 +183,10 @@ def evaluate(model: str, log_n: int = 21, degree: int = 32, panel_products: floa
                      f"{len(panels)} row panels of <= {panel_products:.0e} products",
            "products": products, "nnz_c": nnz_total, "plain_ms": round(t_plain, 2),
            "learned_ms": round(t_learned, 2), "speedup": round(t_plain / t_learned, 3),
+           "plain_output_alloc_ms": round(a_plain, 2),
+           "learned_output_alloc_ms": round(a_learned, 2),
+           "speedup_without_output_alloc":
+               round((t_plain - a_plain) / max(t_learned - a_learned, 1e-9), 3),
            "plain_gflops": round(2 * products / t_plain / 1e6, 2),
            "learned_gflops": round(2 * products / t_learned / 1e6, 2),
            "overflow_rows": int(ovf), "capacity_over_nnz": round(est_total / max(nnz_total, 1), 3),

@@ -171,18 +171,32 @@ def heavy_rows(gpu, log_cols, seed=9):
 
 @pytest.mark.parametrize("log_cols", [20, 21])
 def test_rank1_row_pieces(gpu, log_cols):
-    """Rows of more than 2^21 products with stored bitmaps are split into
-    pieces of consecutive tiles on different workgroups, each starting from
-    the outputs counted in the stored words before it.  Over 2^20 columns
-    the bitmaps come from the dense symbolic kernel; over 2^21 (wider than
-    its LDS bitmap) from the windowed symbolic kernel.  Precise workflow,
-    then a value-only replay of the plan."""
+    """Rows of more than 2^21 products are split into pieces of consecutive
+    tiles on different workgroups.  Precise workflow: each piece starts from
+    the outputs counted in the stored words before it; over 2^20 columns the
+    bitmaps come from the dense symbolic kernel, over 2^21 (wider than its
+    LDS bitmap) from the windowed symbolic kernel; then a value-only replay
+    of the plan.  Estimation workflow (no stored bitmaps): a count pass
+    gives each piece its outputs, and a row over its estimated capacity is
+    listed once for the recount."""
     log(f"case: rank1 row pieces, B 2^15 x 2^{log_cols}")
     A, B = heavy_rows(gpu, log_cols)
     C, st = _run(gpu, A, B, gpu.Config(use_tiled_numeric=3), 0)
     assert st["decision"]["stored_row_bitmap_bytes"] > 0
     worst, _ = check_product(A, B, C, TOL64, f"row pieces 2^{log_cols}")
     log(f"row pieces 2^{log_cols}: nnz {C.nnz}, err {worst:.2e}")
+    for coe in (None, 0.5):  # 0.5: capacities below the counts, rows overflow
+        cfg = gpu.Config(use_tiled_numeric=3)
+        if coe:
+            cfg = gpu.Config(use_tiled_numeric=3, hll_expansion_low=coe, hll_expansion_high=coe)
+        C, st = _run(gpu, A, B, cfg, 1)
+        assert st["decision"]["workflow"] == 1
+        ovf = (st.get("estimation") or {}).get("overflow_rows")
+        if coe:
+            assert ovf and ovf > 0, "no row overflowed its halved capacity"
+        worst, _ = check_product(A, B, C, TOL64, f"row pieces est 2^{log_cols} coe {coe}")
+        log(f"row pieces estimation 2^{log_cols} expansion {coe}: overflow rows {ovf}, "
+            f"err {worst:.2e}")
     dA, dB = A.upload(), B.upload()
     sp = gpu.SpGEMM(gpu.Config(use_tiled_numeric=3))
     sp.set_workflow(0)
