@@ -39,9 +39,8 @@ OutCache &out_cache() {
   static OutCache *c = new OutCache();  // never destroyed: freed at process exit
   return *c;
 }
-// Gives the idle blocks of device dev back to the driver, smallest first
-// (large blocks serve any request), until at most keep bytes stay cached
-// (lock held).
+// Gives the idle blocks of device dev back to the driver, smallest first,
+// until at most keep bytes stay cached (lock held).
 void out_shrink(OutCache &c, int dev, size_t keep) {
   auto &idle = c.idle[dev];
   while (c.idle_bytes[dev] > keep && !idle.empty()) {
@@ -104,11 +103,17 @@ void out_free(void *p) {
   const int dev = b->second.dev;
   c.idle[dev].emplace(b->second.size, p);
   c.idle_bytes[dev] += b->second.size;
-  size_t fr = 0, tot = 0;
-  int cur = 0;
-  if (hipGetDevice(&cur) == hipSuccess && cur == dev && hipMemGetInfo(&fr, &tot) == hipSuccess)
-    out_shrink(c, dev, tot / 4);
-  (void)hipGetLastError();
+}
+
+void out_release() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  OutCache &c = out_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  out_shrink(c, dev, 0);
 }
 
 static thread_local std::string g_err;

@@ -391,12 +391,16 @@ __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a, const uint32_t 
 // bmoff/bm (optional): rows with bmoff[r] != ~0 also store their bitmap,
 // n4 uint4 groups from column left & ~127, at bm + bmoff[r], for the
 // one-gather rank tiles of the numeric pass.
-template <int BS>
+// WIN: the row's span may exceed the LDS bitmap; it is counted in windows
+// of wbits columns (a multiple of 128 the LDS holds), each window sweeping
+// the row's B rows again and keeping the columns inside it (the role of the
+// windowed symbolic pass for rows wider than any dense bin).
+template <int BS, bool WIN = false>
 __device__ __forceinline__ void sym_dense_row(KArgs<float> a,
                                                   uint32_t r,
                                                   uint32_t *__restrict__ nnz,
                                                   const uint64_t *__restrict__ bmoff,
-                                                  uint4 *__restrict__ bm) {
+                                                  uint4 *__restrict__ bm, uint32_t wbits = 0) {
   extern __shared__ uint32_t lds_u32[];
   uint32_t *hdr = lds_u32;
   uint32_t *bits = lds_u32 + 64;
@@ -407,56 +411,65 @@ __device__ __forceinline__ void sym_dense_row(KArgs<float> a,
     return;
   }
   const uint64_t bo = bmoff ? bmoff[r] : ~0ull;
-  // the bitmap is cleared and counted 128 bits at a time (the launch
-  // allocates whole 16-byte groups)
-  const uint32_t nwords = ((hi - lo) >> 5) + 1, n4 = (nwords + 3) / 4;
+  const uint32_t nwin = WIN ? (uint32_t)(((uint64_t)hi - lo) / wbits) + 1 : 1u;
   uint4 *bits4 = reinterpret_cast<uint4 *>(bits);
-  for (uint32_t i = threadIdx.x; i < n4; i += BS) bits4[i] = uint4{0, 0, 0, 0};
   if (threadIdx.x == 0) hdr[0] = hdr[4] = 0;
   __syncthreads();
   const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
   const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
-  auto mark = [&](uint32_t col) {
-    const uint32_t c = col - lo;
-    atomicOr(&bits[c >> 5], 1u << (c & 31));
-  };
   uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
   const uint32_t nheavy =
       list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[4], hdr + 32, kDenseHeavy, &heavy);
-  for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
-    const uint32_t k = a.a_ci[j];
-    const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
-    if (nheavy && be - bs > heavy) continue;
-    seg_cols<8, BS == 1024>(bs, be, lig, G, a.b_ci, mark);
+  for (uint32_t w = 0; w < nwin; ++w) {
+    // window [w0, w1] (the whole span without WIN); the bitmap is cleared
+    // and counted 128 bits at a time (the launch allocates whole 16-byte
+    // groups)
+    const uint32_t w0 = lo + w * wbits;
+    const uint32_t w1 = WIN ? (uint32_t)min((uint64_t)hi, (uint64_t)w0 + wbits - 1) : hi;
+    const uint32_t nwords = ((w1 - w0) >> 5) + 1, n4 = (nwords + 3) / 4;
+    for (uint32_t i = threadIdx.x; i < n4; i += BS) bits4[i] = uint4{0, 0, 0, 0};
+    __syncthreads();
+    auto mark = [&](uint32_t col) {
+      const uint32_t c = col - w0;
+      if (!WIN || (col >= w0 && col <= w1)) atomicOr(&bits[c >> 5], 1u << (c & 31));
+    };
+    for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
+      const uint32_t k = a.a_ci[j];
+      const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
+      if (nheavy && be - bs > heavy) continue;
+      seg_cols<8, BS == 1024>(bs, be, lig, G, a.b_ci, mark);
+    }
+    for (uint32_t h = 0; h < nheavy; ++h) {
+      const uint32_t k = a.a_ci[hdr[32 + h]];
+      seg_cols<8, BS == 1024>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, mark);
+    }
+    __syncthreads();
+    uint32_t cnt = 0;
+    const uint64_t go = bo + (w0 - lo) / 128;  // the window's first stored group
+    for (uint32_t i = threadIdx.x; i < n4; i += BS) {
+      const uint4 v = bits4[i];
+      cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+      if (bo != ~0ull) bm[go + i] = v;
+    }
+    cnt = group_sum(cnt, kWave);
+    if (lane_id() == 0) atomicAdd(&hdr[0], cnt);
+    __syncthreads();  // counted before the next window clears the bitmap
   }
-  for (uint32_t h = 0; h < nheavy; ++h) {
-    const uint32_t k = a.a_ci[hdr[32 + h]];
-    seg_cols<8, BS == 1024>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, mark);
-  }
-  __syncthreads();
-  uint32_t cnt = 0;
-  for (uint32_t i = threadIdx.x; i < n4; i += BS) {
-    const uint4 v = bits4[i];
-    cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
-    if (bo != ~0ull) bm[bo + i] = v;
-  }
-  cnt = group_sum(cnt, kWave);
-  if (lane_id() == 0) atomicAdd(&hdr[0], cnt);
-  __syncthreads();
   if (threadIdx.x == 0) nnz[r] = hdr[0];
 }
 
-template <int BS>
+template <int BS, bool WIN = false>
 __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a, const uint32_t *__restrict__ rows,
-    const uint32_t *__restrict__ bounds, uint32_t *__restrict__ nnz, const uint64_t *__restrict__ bmoff, uint4 *__restrict__ bm) {
+    const uint32_t *__restrict__ bounds, uint32_t *__restrict__ nnz, const uint64_t *__restrict__ bmoff, uint4 *__restrict__ bm,
+    uint32_t wbits = 0) {
   if (!bounds) {
-    sym_dense_row<BS>(a, rows[blockIdx.x], nnz, bmoff, bm);
+    sym_dense_row<BS, WIN>(a, rows[blockIdx.x], nnz, bmoff, bm, wbits);
     return;
   }
   // persistent: the rows [bounds[0], bounds[1]) of the list
   const uint32_t b1 = bounds[1];
   for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
-    sym_dense_row<BS>(a, rows[i], nnz, bmoff, bm);
+    sym_dense_row<BS, WIN>(a, rows[i], nnz, bmoff, bm, wbits);
     __syncthreads();  // the next row re-initialises the LDS
   }
 }
@@ -1608,14 +1621,6 @@ static int window_go(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
   return OC_OK;
 }
 
-int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
-                      uint32_t max_a_nnz, uint32_t *nnz, void *scratch,
-                      hipStream_t s, const RowSel &sel, const uint64_t *bmoff, void *bm) {
-  NumOut o;
-  std::memset(&o, 0, sizeof(o));
-  return window_go<float, true>(a, rows, nrows, max_a_nnz, o, nnz, scratch, s, sel, bmoff, bm);
-}
-
 int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                       uint32_t max_a_nnz, const NumOut &o, void *scratch,
                       hipStream_t s, const RowSel &sel) {
@@ -1696,6 +1701,31 @@ int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
   } while (0)
   BIN_DISPATCH(dense_bin, GO)
 #undef GO
+  LAUNCH_CHECK();
+  return OC_OK;
+}
+
+// Rows wider than every dense bin (and rows whose hash table filled up):
+// the windowed form of the dense symbolic kernel, 1024 threads and the
+// largest LDS bitmap of one workgroup, one window of wbits columns at a
+// time.  max_a_nnz and scratch are unused (the cursor kernel needed them).
+int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
+                      uint32_t max_a_nnz, uint32_t *nnz, void *scratch,
+                      hipStream_t s, const RowSel &sel, const uint64_t *bmoff, void *bm) {
+  (void)max_a_nnz;
+  (void)scratch;
+  if (nrows == 0 && !sel.bounds) return OC_OK;
+  const Tables &t = tables();
+  const size_t lds = std::min<size_t>(t.lds_max_block, t.lds_per_cu) & ~(size_t)15;
+  if (lds < 256 + 3 * 16) {
+    set_error("windowed symbolic kernel: %zu B of LDS is too little", lds);
+    return OC_ERR_INVALID;
+  }
+  const uint32_t wbits = (uint32_t)(((lds - 256) / 16 - 1) * 128);
+  OC_TRY(set_lds(k_sym_dense<1024, true>, lds));
+  const uint32_t grid = sel.bounds ? persistent_grid(k_sym_dense<1024, true>, 1024, lds) : nrows;
+  hipLaunchKernelGGL((k_sym_dense<1024, true>), dim3(grid), dim3(1024), lds, s,
+                     kargs<float>(a), rows, sel.bounds, nnz, bmoff, (uint4 *)bm, wbits);
   LAUNCH_CHECK();
   return OC_OK;
 }

@@ -34,13 +34,17 @@ inline size_t vsize(int vtype) { return vtype == OC_F32 ? 4 : 8; }
 
 // Device buffers of product outputs (C's arrays, oc_csr.owned == 3): freed
 // buffers stay in a per-device cache and are reused best fit (2 MiB units;
-// requests under 64 MiB take buffers of at most four times their size); the
-// cache holds at most a quarter of the device, dropping small buffers first.  Device memory handed back to the driver is
-// cleared before it is handed out again, at tens of GB/s, so a series of
-// differently sized products that freed and allocated its outputs spent up
-// to 1.5 s per product in the allocation.  out_free waits for the device.
+// requests under 64 MiB take buffers of at most four times their size).
+// Device memory handed back to the driver is cleared before it is handed
+// out again, at tens of GB/s, so a series of differently sized products
+// that freed and allocated its outputs spent up to 1.5 s per product in the
+// allocation.  The cached buffers go back to the driver only when a device
+// allocation of the library fails (out_release, then one retry), as the
+// stream-ordered pool keeps freed memory with its release threshold at
+// the maximum.  out_free waits for the device.
 int out_alloc(void **p, size_t bytes);
 void out_free(void *p);
+void out_release();  // the current device's cached buffers back to the driver
 
 // splitmix64: the seeded generator behind every synthetic matrix.
 struct SplitMix64 {

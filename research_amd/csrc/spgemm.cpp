@@ -112,6 +112,11 @@ struct Arena {
     uint8_t *base = nullptr;
     const size_t size = std::max<size_t>(bytes, 64u << 20);
     hipError_t e = hipMalloc((void **)&base, size);
+    if (e == hipErrorOutOfMemory) {  // the cached product outputs back, once
+      (void)hipGetLastError();
+      out_release();
+      e = hipMalloc((void **)&base, size);
+    }
     if (e != hipSuccess) {
       (void)hipGetLastError();
       set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
