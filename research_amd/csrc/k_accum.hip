@@ -276,6 +276,32 @@ __device__ __forceinline__ uint32_t list_heavy(const KA &a, uint32_t r, uint32_t
   return n <= cap ? n : 0u;
 }
 
+// Persistent row loop over the rows [bounds[0], bounds[1]) of a bin list:
+// chunks of kClaimRows rows claimed from *claim (zeroed before the launch).
+// A workgroup that starts late or draws costly rows takes fewer chunks;
+// each workgroup striding over a fixed share of the rows ran the bins
+// 10-40% slower than launching one workgroup per row.  One loop latch,
+// after the barriers that publish the claim.
+constexpr uint32_t kClaimRows = 16;
+template <typename F>
+__device__ __forceinline__ void claim_rows(const uint32_t *__restrict__ bounds, uint32_t *claim,
+                                           F &&row) {
+  __shared__ uint32_t s_claim;
+  const uint32_t b0 = bounds[0], b1 = bounds[1];
+  for (;;) {
+    if (threadIdx.x == 0) s_claim = atomicAdd(claim, kClaimRows);
+    __syncthreads();
+    const uint32_t c = s_claim;
+    __syncthreads();
+    if (c >= b1 - min(b0, b1)) break;
+    const uint32_t e = min(b1, b0 + c + kClaimRows);
+    for (uint32_t i = b0 + c; i < e; ++i) {
+      row(i);
+      __syncthreads();  // the next row re-initialises the LDS
+    }
+  }
+}
+
 // ========================================================== symbolic hash
 constexpr uint32_t kSymHeavy = 56;  // listed heavy B rows per row (hdr[8..63])
 constexpr uint32_t kDenseHeavy = 32;  // dense kernels: hdr[32..63], count hdr[4]
@@ -372,17 +398,15 @@ __device__ __forceinline__ void sym_hash_row(KArgs<float> a,
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_sym_hash(KArgs<float> a, const uint32_t *__restrict__ rows,
-    const uint32_t *__restrict__ bounds, uint32_t S, uint32_t limit, uint32_t *__restrict__ nnz, uint32_t *ovf_rows, uint32_t *ovf_count) {
+    const uint32_t *__restrict__ bounds, uint32_t S, uint32_t limit, uint32_t *__restrict__ nnz, uint32_t *ovf_rows, uint32_t *ovf_count,
+    uint32_t *claim) {
   if (!bounds) {
     sym_hash_row<BS>(a, rows[blockIdx.x], S, limit, nnz, ovf_rows, ovf_count);
     return;
   }
-  // persistent: the rows [bounds[0], bounds[1]) of the list
-  const uint32_t b1 = bounds[1];
-  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
+  claim_rows(bounds, claim, [&](uint32_t i) {
     sym_hash_row<BS>(a, rows[i], S, limit, nnz, ovf_rows, ovf_count);
-    __syncthreads();  // the next row re-initialises the LDS
-  }
+  });
 }
 
 // ========================================================= symbolic dense
@@ -461,17 +485,14 @@ __device__ __forceinline__ void sym_dense_row(KArgs<float> a,
 template <int BS, bool WIN = false>
 __global__ __launch_bounds__(BS) void k_sym_dense(KArgs<float> a, const uint32_t *__restrict__ rows,
     const uint32_t *__restrict__ bounds, uint32_t *__restrict__ nnz, const uint64_t *__restrict__ bmoff, uint4 *__restrict__ bm,
-    uint32_t wbits = 0) {
+    uint32_t wbits, uint32_t *claim) {
   if (!bounds) {
     sym_dense_row<BS, WIN>(a, rows[blockIdx.x], nnz, bmoff, bm, wbits);
     return;
   }
-  // persistent: the rows [bounds[0], bounds[1]) of the list
-  const uint32_t b1 = bounds[1];
-  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
+  claim_rows(bounds, claim, [&](uint32_t i) {
     sym_dense_row<BS, WIN>(a, rows[i], nnz, bmoff, bm, wbits);
-    __syncthreads();  // the next row re-initialises the LDS
-  }
+  });
 }
 
 // ========================================================== numeric hash
@@ -634,18 +655,16 @@ __device__ __forceinline__ void num_hash_row(KArgs<T> a,
 
 template <int BS, typename T, bool EST>
 __global__ __launch_bounds__(BS) void k_num_hash(KArgs<T> a, const uint32_t *__restrict__ rows,
-    const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err, HashLayout L, KOut<T> o, int key_bits) {
+    const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err, HashLayout L, KOut<T> o, int key_bits,
+    uint32_t *claim) {
   if (err && *err) return;  // the capacity check failed: write nothing
   if (!bounds) {
     num_hash_row<BS, T, EST>(a, rows[blockIdx.x], L, o, key_bits);
     return;
   }
-  // persistent: the rows [bounds[0], bounds[1]) of the list
-  const uint32_t b1 = bounds[1];
-  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
+  claim_rows(bounds, claim, [&](uint32_t i) {
     num_hash_row<BS, T, EST>(a, rows[i], L, o, key_bits);
-    __syncthreads();  // the next row re-initialises the LDS
-  }
+  });
 }
 
 // ========================================================= numeric dense
@@ -722,18 +741,14 @@ __device__ __forceinline__ void num_dense_row(KArgs<T> a,
 
 template <int BS, typename T, bool EST>
 __global__ __launch_bounds__(BS) void k_num_dense(KArgs<T> a, const uint32_t *__restrict__ rows,
-    const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err, KOut<T> o) {
+    const uint32_t *__restrict__ bounds, const uint32_t *__restrict__ err, KOut<T> o,
+    uint32_t *claim) {
   if (err && *err) return;  // the capacity check failed: write nothing
   if (!bounds) {
     num_dense_row<BS, T, EST>(a, rows[blockIdx.x], o);
     return;
   }
-  // persistent: the rows [bounds[0], bounds[1]) of the list
-  const uint32_t b1 = bounds[1];
-  for (uint32_t i = bounds[0] + blockIdx.x; i < b1; i += gridDim.x) {
-    num_dense_row<BS, T, EST>(a, rows[i], o);
-    __syncthreads();  // the next row re-initialises the LDS
-  }
+  claim_rows(bounds, claim, [&](uint32_t i) { num_dense_row<BS, T, EST>(a, rows[i], o); });
 }
 
 // ============================================================== windowed
@@ -1674,9 +1689,10 @@ int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
   do {                                                                        \
     OC_TRY(set_lds(k_sym_hash<BS>, lds));                                     \
     const uint32_t grid = sel.bounds ? persistent_grid(k_sym_hash<BS>, BS, lds) : nrows; \
+    if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));                        \
     hipLaunchKernelGGL(k_sym_hash<BS>, dim3(grid), dim3(BS), lds, s, k, rows,  \
                        sel.bounds, S, std::min(limit, S), nnz, ovf_rows,      \
-                       ovf_count);                                            \
+                       ovf_count, sel.claim);                                 \
   } while (0)
   BIN_DISPATCH(bin, GO)
 #undef GO
@@ -1696,8 +1712,9 @@ int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
   do {                                                                         \
     OC_TRY(set_lds(k_sym_dense<BS>, lds));                                     \
     const uint32_t grid = sel.bounds ? persistent_grid(k_sym_dense<BS>, BS, lds) : nrows; \
+    if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));                         \
     hipLaunchKernelGGL(k_sym_dense<BS>, dim3(grid), dim3(BS), lds, s, k, rows,  \
-                       sel.bounds, nnz, bmoff, (uint4 *)bm);                   \
+                       sel.bounds, nnz, bmoff, (uint4 *)bm, 0u, sel.claim);    \
   } while (0)
   BIN_DISPATCH(dense_bin, GO)
 #undef GO
@@ -1716,7 +1733,8 @@ int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
   (void)scratch;
   if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
-  const size_t lds = std::min<size_t>(t.lds_max_block, t.lds_per_cu) & ~(size_t)15;
+  // (256 B left for the kernel's static LDS: the row-claim slot)
+  const size_t lds = (std::min<size_t>(t.lds_max_block, t.lds_per_cu) - 256) & ~(size_t)15;
   if (lds < 256 + 3 * 16) {
     set_error("windowed symbolic kernel: %zu B of LDS is too little", lds);
     return OC_ERR_INVALID;
@@ -1724,8 +1742,10 @@ int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
   const uint32_t wbits = (uint32_t)(((lds - 256) / 16 - 1) * 128);
   OC_TRY(set_lds(k_sym_dense<1024, true>, lds));
   const uint32_t grid = sel.bounds ? persistent_grid(k_sym_dense<1024, true>, 1024, lds) : nrows;
+  if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));
   hipLaunchKernelGGL((k_sym_dense<1024, true>), dim3(grid), dim3(1024), lds, s,
-                     kargs<float>(a), rows, sel.bounds, nnz, bmoff, (uint4 *)bm, wbits);
+                     kargs<float>(a), rows, sel.bounds, nnz, bmoff, (uint4 *)bm, wbits,
+                     sel.claim);
   LAUNCH_CHECK();
   return OC_OK;
 }
@@ -1745,9 +1765,10 @@ int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
   do {                                                                         \
     OC_TRY(set_lds(k_num_hash<BS, T, E>, L.bytes));                            \
     const uint32_t grid = sel.bounds ? persistent_grid(k_num_hash<BS, T, E>, BS, L.bytes) : nrows; \
+    if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));                         \
     hipLaunchKernelGGL((k_num_hash<BS, T, E>), dim3(grid), dim3(BS), L.bytes,  \
                        s, kargs<T>(a), rows, sel.bounds, sel.err, L,           \
-                       kout<T>(o), kb);                                        \
+                       kout<T>(o), kb, sel.claim);                             \
   } while (0)
 #define GO(BS)                                            \
   do {                                                    \
@@ -1782,8 +1803,10 @@ int launch_num_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
   do {                                                                          \
     OC_TRY(set_lds(k_num_dense<BS, T, E>, lds));                                \
     const uint32_t grid = sel.bounds ? persistent_grid(k_num_dense<BS, T, E>, BS, lds) : nrows; \
+    if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));                          \
     hipLaunchKernelGGL((k_num_dense<BS, T, E>), dim3(grid), dim3(BS), lds, s,   \
-                       kargs<T>(a), rows, sel.bounds, sel.err, kout<T>(o));     \
+                       kargs<T>(a), rows, sel.bounds, sel.err, kout<T>(o),      \
+                       sel.claim);                                              \
   } while (0)
 #define GO(BS)                                            \
   do {                                                    \

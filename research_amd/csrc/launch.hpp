@@ -160,6 +160,7 @@ struct NumOut {
 struct RowSel {
   const uint32_t *bounds = nullptr;
   const uint32_t *err = nullptr;
+  uint32_t *claim = nullptr;  // row-claim counter of the hash/dense bins (zeroed per launch)
 };
 
 // Symbolic: rows of one bin.  Overflowing rows (estimated-compaction hash

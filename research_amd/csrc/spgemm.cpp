@@ -1897,6 +1897,9 @@ int full_replay_impl(oc_spgemm *h, hipStream_t s) {
   r.win_scratch = p.win_scratch;
   const uint64_t m = p.A.rows;
   uint32_t *err = p.err;
+  // err[0]: check bits; err[8..9]: bounds of the symbolic overflow list;
+  // err[32]: the row-claim counter of the persistent bin kernels
+  uint32_t *claim = err + 32;
   const AccArgs a = r.acc();
   const Tables &t = tables();
   OC_TRY(launch_fill_u32(err, 1, 0u, s));
@@ -1916,28 +1919,28 @@ int full_replay_impl(oc_spgemm *h, hipStream_t s) {
   for (int b = 0; b <= 5; ++b) {
     const uint32_t S = t.hash_sym[b];
     OC_TRY(launch_sym_hash(b, a, p.brows1, 0, p.est_symbolic ? S - S / 8 : S, p.nnz, ovf_rows,
-                           p.ovf, s, RowSel{p.boff1 + b, nullptr}));
+                           p.ovf, s, RowSel{p.boff1 + b, nullptr, claim}));
   }
   OC_TRY(launch_sym_window(a, p.brows1, 0, p.max_a_nnz, p.nnz, p.win_scratch, s,
-                           RowSel{p.boff1 + 6, err}));
+                           RowSel{p.boff1 + 6, err, claim}));
   for (int b = 8; b <= 13; ++b)
     OC_TRY(launch_sym_dense(b - 7, a, p.brows1, 0, p.nnz, s, nullptr, nullptr,
-                            RowSel{p.boff1 + b, nullptr}));
+                            RowSel{p.boff1 + b, nullptr, claim}));
   // rows whose table filled up are recounted by the windowed kernel over
   // the device-side overflow list [0, ovf[0])
   uint32_t *ovf_bounds = err + 8;
   OC_TRY(launch_count_bounds(p.ovf, ovf_bounds, s));
   OC_TRY(launch_sym_window(a, ovf_rows, 0, p.max_a_nnz, p.nnz, p.win_scratch, s,
-                           RowSel{ovf_bounds, err}));
+                           RowSel{ovf_bounds, err, claim}));
   // C's offsets, and its capacity against the new nnz
   OC_TRY(launch_scan_u32(p.nnz, (uint64_t *)p.o.off, m, p.scan_tmp, s));
   OC_TRY(launch_check_gt(p.o.off + m, 1, p.cap_nnz, kFullCapacity, err, s));
   // numeric: binning on the exact counts, then every bin persistent
   OC_TRY(launch_binning(p.num_bp, m, p.bor2, const_cast<uint32_t *>(p.brows), p.boff2, p.bin_tmp, s));
   for (int b = 1; b <= 6; ++b)
-    OC_TRY(launch_num_hash(b, a, p.brows, 0, p.o, s, RowSel{p.boff2 + b, err}));
+    OC_TRY(launch_num_hash(b, a, p.brows, 0, p.o, s, RowSel{p.boff2 + b, err, claim}));
   for (int b = 7; b <= 12; ++b)
-    OC_TRY(launch_num_dense(b - 7, a, p.brows, 0, p.o, s, RowSel{p.boff2 + b, err}));
+    OC_TRY(launch_num_dense(b - 7, a, p.brows, 0, p.o, s, RowSel{p.boff2 + b, err, claim}));
   const uint32_t keep_max = h->max_a_nnz;
   h->max_a_nnz = p.max_a_nnz;
   int rc = r.big_rows(p.brows, 0, p.o, RowSel{p.boff2 + 13, err});
