@@ -286,12 +286,18 @@ constexpr uint32_t kClaimRows = 16;
 template <typename F>
 __device__ __forceinline__ void claim_rows(const uint32_t *__restrict__ bounds, uint32_t *claim,
                                            F &&row) {
-  __shared__ uint32_t s_claim;
+  // The claim is published in the first word of the dynamic LDS, which no
+  // row keeps between rows (each row initialises its LDS).  A static slot
+  // would add LDS to every launch of the kernel: 4 bytes moved the dynamic
+  // LDS off its 16-byte alignment (k_sym_dense 32.5 -> 37.2 ms), and 16
+  // bytes took a workgroup per CU from bins sized to the CU's LDS
+  // (k_sym_hash 4.6 -> 6.4 ms on R-MAT 2^20).
+  extern __shared__ uint32_t s_claim[];
   const uint32_t b0 = bounds[0], b1 = bounds[1];
   for (;;) {
-    if (threadIdx.x == 0) s_claim = atomicAdd(claim, kClaimRows);
+    if (threadIdx.x == 0) s_claim[0] = atomicAdd(claim, kClaimRows);
     __syncthreads();
-    const uint32_t c = s_claim;
+    const uint32_t c = s_claim[0];
     __syncthreads();
     if (c >= b1 - min(b0, b1)) break;
     const uint32_t e = min(b1, b0 + c + kClaimRows);
@@ -1677,6 +1683,27 @@ static uint32_t persistent_grid(K kernel, uint32_t bs, size_t lds) {
   return std::max<uint32_t>(1, t.num_cu) * (uint32_t)per_cu;
 }
 
+// The grid of a persistent launch over a bin: every resident workgroup, or
+// one per kClaimRows rows when the host knows the bin's size (nrows > 0).
+template <typename K>
+static uint32_t claim_grid(K kernel, uint32_t bs, size_t lds, uint32_t nrows) {
+  const uint32_t full = persistent_grid(kernel, bs, lds);
+  return nrows ? std::min<uint32_t>(full, cdiv(nrows, kClaimRows)) : full;
+}
+
+// A bin launch whose host knows the bin (sel.first, nrows) runs one
+// workgroup per row when more than two workgroups fit a CU: the hardware
+// then hides the gap between one row's workgroup and the next, and the
+// claims cost more than they gain (R-MAT 2^20: k_num_dense 0.11 -> 0.46 ms
+// persistent).  With one or two per CU the CU idles in that gap, and the
+// claims win (k_sym_dense<1024> 32.6 -> 26.1 ms).
+template <typename K>
+static bool run_persistent(const RowSel &sel, uint32_t nrows, K kernel, uint32_t bs, size_t lds) {
+  if (!sel.bounds) return false;
+  if (!nrows || sel.first == ~0u) return true;
+  return persistent_grid(kernel, bs, lds) <= 2 * std::max<uint32_t>(1, tables().num_cu);
+}
+
 int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
                     uint32_t nrows, uint32_t limit, uint32_t *nnz,
                     uint32_t *ovf_rows, uint32_t *ovf_count, hipStream_t s, const RowSel &sel) {
@@ -1688,11 +1715,13 @@ int launch_sym_hash(int bin, const AccArgs &a, const uint32_t *rows,
 #define GO(BS)                                                                \
   do {                                                                        \
     OC_TRY(set_lds(k_sym_hash<BS>, lds));                                     \
-    const uint32_t grid = sel.bounds ? persistent_grid(k_sym_hash<BS>, BS, lds) : nrows; \
-    if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));                        \
-    hipLaunchKernelGGL(k_sym_hash<BS>, dim3(grid), dim3(BS), lds, s, k, rows,  \
-                       sel.bounds, S, std::min(limit, S), nnz, ovf_rows,      \
-                       ovf_count, sel.claim);                                 \
+    const bool per = run_persistent(sel, nrows, k_sym_hash<BS>, BS, lds);    \
+    const uint32_t grid = per ? claim_grid(k_sym_hash<BS>, BS, lds, nrows) : nrows; \
+    if (per) OC_TRY(zero_u32(sel.claim, 1, s));                               \
+    hipLaunchKernelGGL(k_sym_hash<BS>, dim3(grid), dim3(BS), lds, s, k,        \
+                       per || !sel.bounds ? rows : rows + sel.first,          \
+                       per ? sel.bounds : nullptr, S, std::min(limit, S), nnz, \
+                       ovf_rows, ovf_count, sel.claim);                       \
   } while (0)
   BIN_DISPATCH(bin, GO)
 #undef GO
@@ -1711,10 +1740,13 @@ int launch_sym_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
 #define GO(BS)                                                                 \
   do {                                                                         \
     OC_TRY(set_lds(k_sym_dense<BS>, lds));                                     \
-    const uint32_t grid = sel.bounds ? persistent_grid(k_sym_dense<BS>, BS, lds) : nrows; \
-    if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));                         \
-    hipLaunchKernelGGL(k_sym_dense<BS>, dim3(grid), dim3(BS), lds, s, k, rows,  \
-                       sel.bounds, nnz, bmoff, (uint4 *)bm, 0u, sel.claim);    \
+    const bool per = run_persistent(sel, nrows, k_sym_dense<BS>, BS, lds);     \
+    const uint32_t grid = per ? claim_grid(k_sym_dense<BS>, BS, lds, nrows) : nrows; \
+    if (per) OC_TRY(zero_u32(sel.claim, 1, s));                                \
+    hipLaunchKernelGGL(k_sym_dense<BS>, dim3(grid), dim3(BS), lds, s, k,        \
+                       per || !sel.bounds ? rows : rows + sel.first,           \
+                       per ? sel.bounds : nullptr, nnz, bmoff, (uint4 *)bm, 0u, \
+                       sel.claim);                                             \
   } while (0)
   BIN_DISPATCH(dense_bin, GO)
 #undef GO
@@ -1733,19 +1765,19 @@ int launch_sym_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
   (void)scratch;
   if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
-  // (256 B left for the kernel's static LDS: the row-claim slot)
-  const size_t lds = (std::min<size_t>(t.lds_max_block, t.lds_per_cu) - 256) & ~(size_t)15;
+  const size_t lds = std::min<size_t>(t.lds_max_block, t.lds_per_cu) & ~(size_t)15;
   if (lds < 256 + 3 * 16) {
     set_error("windowed symbolic kernel: %zu B of LDS is too little", lds);
     return OC_ERR_INVALID;
   }
   const uint32_t wbits = (uint32_t)(((lds - 256) / 16 - 1) * 128);
   OC_TRY(set_lds(k_sym_dense<1024, true>, lds));
-  const uint32_t grid = sel.bounds ? persistent_grid(k_sym_dense<1024, true>, 1024, lds) : nrows;
-  if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));
+  const bool per = run_persistent(sel, nrows, k_sym_dense<1024, true>, 1024, lds);
+  const uint32_t grid = per ? claim_grid(k_sym_dense<1024, true>, 1024, lds, nrows) : nrows;
+  if (per) OC_TRY(zero_u32(sel.claim, 1, s));
   hipLaunchKernelGGL((k_sym_dense<1024, true>), dim3(grid), dim3(1024), lds, s,
-                     kargs<float>(a), rows, sel.bounds, nnz, bmoff, (uint4 *)bm, wbits,
-                     sel.claim);
+                     kargs<float>(a), per || !sel.bounds ? rows : rows + sel.first,
+                     per ? sel.bounds : nullptr, nnz, bmoff, (uint4 *)bm, wbits, sel.claim);
   LAUNCH_CHECK();
   return OC_OK;
 }
@@ -1764,11 +1796,13 @@ int launch_num_hash(int bin, const AccArgs &a, const uint32_t *rows,
 #define GO3(BS, T, E)                                                          \
   do {                                                                         \
     OC_TRY(set_lds(k_num_hash<BS, T, E>, L.bytes));                            \
-    const uint32_t grid = sel.bounds ? persistent_grid(k_num_hash<BS, T, E>, BS, L.bytes) : nrows; \
-    if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));                         \
+    const bool per = run_persistent(sel, nrows, k_num_hash<BS, T, E>, BS, L.bytes); \
+    const uint32_t grid = per ? claim_grid(k_num_hash<BS, T, E>, BS, L.bytes, nrows) : nrows; \
+    if (per) OC_TRY(zero_u32(sel.claim, 1, s));                                \
     hipLaunchKernelGGL((k_num_hash<BS, T, E>), dim3(grid), dim3(BS), L.bytes,  \
-                       s, kargs<T>(a), rows, sel.bounds, sel.err, L,           \
-                       kout<T>(o), kb, sel.claim);                             \
+                       s, kargs<T>(a), per || !sel.bounds ? rows : rows + sel.first, \
+                       per ? sel.bounds : nullptr, sel.err, L, kout<T>(o), kb, \
+                       sel.claim);                                             \
   } while (0)
 #define GO(BS)                                            \
   do {                                                    \
@@ -1802,11 +1836,12 @@ int launch_num_dense(int dense_bin, const AccArgs &a, const uint32_t *rows,
 #define GO3(BS, T, E)                                                           \
   do {                                                                          \
     OC_TRY(set_lds(k_num_dense<BS, T, E>, lds));                                \
-    const uint32_t grid = sel.bounds ? persistent_grid(k_num_dense<BS, T, E>, BS, lds) : nrows; \
-    if (sel.bounds) OC_TRY(zero_u32(sel.claim, 1, s));                          \
+    const bool per = run_persistent(sel, nrows, k_num_dense<BS, T, E>, BS, lds); \
+    const uint32_t grid = per ? claim_grid(k_num_dense<BS, T, E>, BS, lds, nrows) : nrows; \
+    if (per) OC_TRY(zero_u32(sel.claim, 1, s));                                 \
     hipLaunchKernelGGL((k_num_dense<BS, T, E>), dim3(grid), dim3(BS), lds, s,   \
-                       kargs<T>(a), rows, sel.bounds, sel.err, kout<T>(o),      \
-                       sel.claim);                                              \
+                       kargs<T>(a), per || !sel.bounds ? rows : rows + sel.first, \
+                       per ? sel.bounds : nullptr, sel.err, kout<T>(o), sel.claim); \
   } while (0)
 #define GO(BS)                                            \
   do {                                                    \

@@ -161,6 +161,10 @@ struct RowSel {
   const uint32_t *bounds = nullptr;
   const uint32_t *err = nullptr;
   uint32_t *claim = nullptr;  // row-claim counter of the hash/dense bins (zeroed per launch)
+  // the bin's first list entry when the host knows it (eager runs, which
+  // pass the bin's row count): bins with many workgroups per CU then run
+  // one workgroup per row from rows + first
+  uint32_t first = ~0u;
 };
 
 // Symbolic: rows of one bin.  Overflowing rows (estimated-compaction hash

@@ -461,6 +461,13 @@ struct Run {
     return a;
   }
 
+  // the row-claim counter of persistent bin launches (zeroed per launch)
+  uint32_t *claim_ctr = nullptr;
+  int claims() {
+    if (!claim_ctr) OC_TRY(ar.get(&claim_ctr, 64));
+    return OC_OK;
+  }
+
   int window_scratch() {
     if (!win_scratch) {
       uint8_t *p;
@@ -681,9 +688,12 @@ struct Run {
   }
 
   // Numeric kernels over numeric bins 1..13 (bin 13 = windowed).
+  // boff (optional): the bin offsets on the device; the hash and dense
+  // bins then run persistent, claiming rows in chunks (claim_ctr)
   int numeric_bins(const uint32_t *brows, const uint32_t *hoff, const NumOut &o,
-                   bool window_bin13) {
+                   bool window_bin13, const uint32_t *boff = nullptr) {
     const AccArgs a = acc();
+    if (boff) OC_TRY(claims());
     for (int b = 0; b <= 6; ++b) {
       const uint32_t n = hoff[b + 1] - hoff[b];
       if (!n) continue;
@@ -691,13 +701,21 @@ struct Run {
         set_error("internal: numeric hash bin 0 is not used");
         return OC_ERR_INVALID;
       }
-      OC_TRY(launch_num_hash(b, a, brows + hoff[b], n, o, s));
+      if (boff)
+        OC_TRY(launch_num_hash(b, a, brows, n, o, s, RowSel{boff + b, nullptr, claim_ctr, hoff[b]}));
+      else
+        OC_TRY(launch_num_hash(b, a, brows + hoff[b], n, o, s));
     }
     fam("num_hash", "numeric_hash", true, brows + hoff[1], hoff[7] - hoff[1]);
     mk.mark("numeric_hash");
     for (int b = 7; b <= 12; ++b) {
       const uint32_t n = hoff[b + 1] - hoff[b];
-      if (n) OC_TRY(launch_num_dense(b - 7, a, brows + hoff[b], n, o, s));
+      if (!n) continue;
+      if (boff)
+        OC_TRY(launch_num_dense(b - 7, a, brows, n, o, s,
+                                RowSel{boff + b, nullptr, claim_ctr, hoff[b]}));
+      else
+        OC_TRY(launch_num_dense(b - 7, a, brows + hoff[b], n, o, s));
     }
     fam("num_dense", "numeric_dense", true, brows + hoff[7], hoff[13] - hoff[7]);
     mk.mark("numeric_dense");
@@ -1091,19 +1109,21 @@ int Run::run_precise() {
   const AccArgs a = acc();
   const Tables &t = tables();
   OC_TRY(row_bitmaps(brows, hoff));
+  OC_TRY(claims());
   for (int b = 0; b <= 5; ++b) {
     const uint32_t n = hoff[b + 1] - hoff[b];
     if (!n) continue;
     const uint32_t S = t.hash_sym[b];
     const uint32_t limit = d.use_est_symbolic ? S - S / 8 : S;
-    OC_TRY(launch_sym_hash(b, a, brows + hoff[b], n, limit, nnz, ovf_rows, ovf, s));
+    OC_TRY(launch_sym_hash(b, a, brows, n, limit, nnz, ovf_rows, ovf, s,
+                           RowSel{boff + b, nullptr, claim_ctr, hoff[b]}));
   }
   fam("sym_hash", "symbolic_hash", false, brows + hoff[0], hoff[6] - hoff[0]);
   mk.mark("symbolic_hash");
   if (hoff[7] > hoff[6]) {
     OC_TRY(window_scratch());
-    OC_TRY(launch_sym_window(a, brows + hoff[6], hoff[7] - hoff[6], h->max_a_nnz, nnz,
-                             win_scratch, s, RowSel{}, row_bmoff, row_bm));
+    OC_TRY(launch_sym_window(a, brows, hoff[7] - hoff[6], h->max_a_nnz, nnz, win_scratch, s,
+                             RowSel{boff + 6, nullptr, claim_ctr, hoff[6]}, row_bmoff, row_bm));
   }
   fam("sym_window", "symbolic_window", false, brows + hoff[6], hoff[7] - hoff[6]);
   mk.mark("symbolic_window");
@@ -1114,7 +1134,8 @@ int Run::run_precise() {
       set_error("internal: dense symbolic bin 0 is not used");
       return OC_ERR_INVALID;
     }
-    OC_TRY(launch_sym_dense(b - 7, a, brows + hoff[b], n, nnz, s, row_bmoff, row_bm));
+    OC_TRY(launch_sym_dense(b - 7, a, brows, n, nnz, s, row_bmoff, row_bm,
+                            RowSel{boff + b, nullptr, claim_ctr, hoff[b]}));
   }
   fam("sym_dense", "symbolic_dense", false, brows + hoff[7], hoff[14] - hoff[7]);
   mk.mark("symbolic_dense");
@@ -1159,7 +1180,7 @@ int Run::run_precise() {
   o.ovf_rows = ovf_rows;
   o.ovf_count = ovf + 1;
   o.nnz_hint = nnz;
-  OC_TRY(numeric_bins(brows2, hoff, o, true));
+  OC_TRY(numeric_bins(brows2, hoff, o, true, boff2));
   OC_TRY(band_numeric(o));
   // exact capacities cannot overflow; recount defensively if one did
   uint32_t n = 0;
@@ -1845,7 +1866,8 @@ int replay_impl(oc_spgemm *h, hipStream_t s) {
   r.band.pref = p.band_pref;
   const uint32_t keep_max = h->max_a_nnz;
   h->max_a_nnz = p.max_a_nnz;
-  int rc = r.numeric_bins(p.brows, p.hoff, p.o, true);
+  r.claim_ctr = p.err + 32;  // the plan's claim counter (no allocation here)
+  int rc = r.numeric_bins(p.brows, p.hoff, p.o, true, p.boff2);
   if (rc == OC_OK) rc = r.band_numeric(p.o);
   h->max_a_nnz = keep_max;
   if (rc == OC_OK && !r.ar.recs.empty()) {
