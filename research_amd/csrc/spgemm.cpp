@@ -1064,7 +1064,7 @@ int Run::run_usparse() {
     OC_TRY(binning(bp, &bor, &brows, &boff, hoff));
     std::memcpy(h->bin2, hoff, sizeof(hoff));
     h->have_bin2 = true;
-    OC_TRY(numeric_bins(brows, hoff, o, true));
+    OC_TRY(numeric_bins(brows, hoff, o, true, boff));
     uint32_t novf = 0;
     OC_TRY(readback(&novf, ovf, 1, s));
     if (novf) {
@@ -1326,7 +1326,7 @@ int Run::run_est() {
   o.ovf_rows = ovf_rows;
   o.ovf_count = ovf;
   o.nnz_hint = est;
-  OC_TRY(numeric_bins(brows, hoff, o, false));
+  OC_TRY(numeric_bins(brows, hoff, o, false, boff));
   // Long rows (bin 13): the rank-tile kernel checks the estimate and works
   // in place; the other long-row kernels have no capacity check, so their
   // rows go to the spare region with the overflow rows.
