@@ -621,9 +621,56 @@ __device__ __forceinline__ void num_hash_row(KArgs<T> a,
   }
   if (ovf) hdr[1] = 1;
   __syncthreads();
+  const uint32_t lo = a.left[r];
+  // Output order by rank, no sort: when a bitmap of the row's column span
+  // and one prefix per word (8 bytes per 32 columns) fit in the sort's
+  // scratch, each key's place in the output is the popcount of the set
+  // bits before it.  Wider rows compact the occupied slots and sort.
+  const uint32_t spanw = a.right[r] >= lo ? (a.right[r] - lo) / 32 + 1 : 0u;
+  if (!hdr[1] && spanw && 8ull * spanw <= L.bytes - L.off_ck) {
+    uint32_t *bits = reinterpret_cast<uint32_t *>(smem + L.off_ck);
+    uint32_t *pref = bits + spanw;
+    for (uint32_t i = threadIdx.x; i < spanw; i += BS) bits[i] = 0u;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < S; i += BS) {
+      const uint32_t kk = keys[i];
+      if (kk != kEmpty) {
+        const uint32_t c = kk - lo;
+        atomicOr(&bits[c >> 5], 1u << (c & 31));
+      }
+    }
+    __syncthreads();
+    // word prefixes: each thread a contiguous run of words
+    const uint32_t per = (spanw + BS - 1) / BS;
+    const uint32_t w0 = min(spanw, threadIdx.x * per), w1 = min(spanw, w0 + per);
+    uint32_t mine = 0;
+    for (uint32_t w = w0; w < w1; ++w) mine += __popc(bits[w]);
+    uint32_t n;
+    uint32_t run = block_excl_sum(mine, dsum, &n);
+    for (uint32_t w = w0; w < w1; ++w) {
+      pref[w] = run;
+      run += __popc(bits[w]);
+    }
+    if (n > L.nmax || (EST && n > o.cap[r])) {  // as the sorting path
+      if (threadIdx.x == 0) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
+      return;
+    }
+    __syncthreads();
+    const uint64_t off = o.off[r];
+    for (uint32_t i = threadIdx.x; i < S; i += BS) {
+      const uint32_t kk = keys[i];
+      if (kk != kEmpty) {
+        const uint32_t c = kk - lo, w = c >> 5;
+        const uint32_t pos = pref[w] + __popc(bits[w] & ((1u << (c & 31)) - 1u));
+        o.cols[off + pos] = kk;
+        o.vals[off + pos] = (T)vals[i];
+      }
+    }
+    if (o.nnz && threadIdx.x == 0) o.nnz[r] = n;
+    return;
+  }
   // compaction of occupied slots; keys relative to the row's first column,
   // so the sort needs only the bits of the row's column span
-  const uint32_t lo = a.left[r];
   if (!hdr[1]) {
     for (uint32_t i = threadIdx.x; i < S; i += BS) {
       const uint32_t kk = keys[i];
