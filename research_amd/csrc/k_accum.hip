@@ -326,13 +326,51 @@ __device__ __forceinline__ void sym_hash_row(KArgs<float> a,
   extern __shared__ uint32_t lds_u32[];
   uint32_t *hdr = lds_u32;
   uint32_t *tab = lds_u32 + 64;
-  for (uint32_t i = threadIdx.x; i < S; i += BS) tab[i] = kEmpty;
+  const bool track = limit < S;
+  // With full-size tables (no load limit), a row whose column span fits the
+  // table's S words as a bitmap is counted with ORs and popcounts instead
+  // of probing; both give the exact count.  Shrunken tables keep hashing,
+  // so the rows that outgrow them are listed as before.
+  const uint32_t blo = a.left[r] & ~31u, bhi = a.right[r];
+  const uint32_t bw = bhi >= a.left[r] ? ((bhi - blo) >> 5) + 1 : 0u;
+  const bool use_bm = !track && bw && bw <= S;
+  if (use_bm) {
+    for (uint32_t i = threadIdx.x; i < bw; i += BS) tab[i] = 0u;
+  } else {
+    for (uint32_t i = threadIdx.x; i < S; i += BS) tab[i] = kEmpty;
+  }
   if (threadIdx.x < 4) hdr[threadIdx.x] = 0;
   __syncthreads();
   const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
   const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
   const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
-  const bool track = limit < S;
+  if (use_bm) {
+    auto mark = [&](uint32_t col) {
+      const uint32_t c = col - blo;
+      atomicOr(&tab[c >> 5], 1u << (c & 31));
+    };
+    uint32_t heavy;
+    const uint32_t nheavy =
+        list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
+    for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
+      const uint32_t k = a.a_ci[j];
+      const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
+      if (nheavy && be - bs > heavy) continue;  // listed: swept below
+      seg_cols<8, false>(bs, be, lig, G, a.b_ci, mark);
+    }
+    for (uint32_t h = 0; h < nheavy; ++h) {
+      const uint32_t k = a.a_ci[hdr[8 + h]];
+      seg_cols<8, false>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, mark);
+    }
+    __syncthreads();
+    uint32_t cnt = 0;
+    for (uint32_t i = threadIdx.x; i < bw; i += BS) cnt += __popc(tab[i]);
+    cnt = group_sum(cnt, kWave);
+    if (lane_id() == 0) atomicAdd(&hdr[2], cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) nnz[r] = hdr[2];
+    return;
+  }
   uint32_t mine = 0;
   bool ovf = false;
   // linear probing from `slot` (`probes` slots already tried), each probe
