@@ -152,6 +152,18 @@ def run_batched(ra, ocean, count: int, steps: int, model_path: str | None):
     return out
 
 
+def compulsory(A, B, st, dtype: str, ms: float) -> dict:
+    """The compulsory traffic of one product: A and B read once, C written
+    once (u32 offsets and columns of the operands, u64 offsets of C), and
+    the share of 8 TB/s it is at the measured step time."""
+    s = 8 if dtype == "f64" else 4
+    ops = sum((M.rows + 1) * 4 + M.nnz * (4 + s) for M in (A, B))
+    c = (st["C"]["rows"] + 1) * 8 + st["C"]["nnz"] * (4 + s)
+    b = ops + c
+    return {"bytes": int(b), "achieved_gbps": round(b / (ms * 1e-3) / 1e9, 2),
+            "frac": round(b / (ms * 1e-3) / 8e12, 4)}
+
+
 def measured_traffic(workload: str, family: str):
     """HBM bytes per launch of the dominant kernel from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs summarised under profiles/ (a line
@@ -367,8 +379,10 @@ def main():
             "bytes_alg": whole.get("bytes_alg"),
             "achieved_gbps": round(whole.get("bytes_alg", 0) / (ms_per_step * 1e-3) / 1e9, 2),
             "frac": round(whole.get("bytes_alg", 0) / (ms_per_step * 1e-3) / 8e12, 4)},
+        "compulsory_model": compulsory(A, B, st, dtype, ms_per_step),
         "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in fam_ms.items()},
         "library_ms_per_step": round(float(np.mean(step_ms)), 3),
+        "library_ms_per_step_median": round(float(np.median(step_ms)), 3),
         "structure_invalid_rows": bad,
     }
     if not args.no_cpu_baseline:
