@@ -843,24 +843,25 @@ __global__ __launch_bounds__(BS) void k_num_dense(KArgs<T> a, const uint32_t *__
 }
 
 // ============================================================== windowed
-// Persistent kernels (one 1024-thread workgroup per CU) over the rows of
-// a bin, taken from an atomic counter.  The row's column span is swept in
-// windows of W columns; each A entry keeps a cursor into its B row, whose
-// columns ascend, so a window consumes a prefix of what is left of every B
-// row.  The cursor lives with the lane group that owns the A entry (LDS for
-// rows up to 4096 entries, else a per-workgroup global scratch slot).
-// Numeric windows hold FP64 sums + a bitmap and are written out in order
-// (kernels/AccumulatorDense.cuh:610-806); symbolic windows only count.
+// The numeric fallback for long rows (use_tiled_numeric 0, or no tile
+// table): persistent, one 1024-thread workgroup per CU, rows taken from an
+// atomic counter.  The row's column span is swept in windows of W columns;
+// each A entry keeps a cursor into its B row, whose columns ascend, so a
+// window consumes a prefix of what is left of every B row.  The cursor
+// lives with the lane group that owns the A entry (LDS for rows up to 4096
+// entries, else a per-workgroup global scratch slot).  Windows hold FP64
+// sums + a bitmap and are written out in order
+// (kernels/AccumulatorDense.cuh:610-806).  (Its symbolic form was replaced
+// by the windowed form of k_sym_dense.)
 constexpr int kWinBlock = 1024;
 constexpr uint32_t kWinLdsCursors = 4096;
 
-template <typename T, bool SYM>
+template <typename T>
 __global__ __launch_bounds__(kWinBlock) void k_window(
     KArgs<T> a, const uint32_t *__restrict__ rows, uint32_t nrows,
-    uint32_t *counter, uint32_t W, KOut<T> o, uint32_t *__restrict__ nnz_sym,
+    uint32_t *counter, uint32_t W, KOut<T> o,
     uint32_t *__restrict__ scratch, uint32_t max_nA, const uint32_t *__restrict__ bounds,
-    const uint32_t *__restrict__ err, const uint64_t *__restrict__ bmoff,
-    uint32_t *__restrict__ bm32) {
+    const uint32_t *__restrict__ err) {
   if (err && *err) return;  // a capacity check of the run failed
   if (bounds) {             // rows [bounds[0], bounds[1]) of the list
     rows += bounds[0];
@@ -872,8 +873,7 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
   uint32_t *bits = reinterpret_cast<uint32_t *>(smem + 256 + kWinLdsCursors * 4);
   double *vals = reinterpret_cast<double *>(smem + 256 + kWinLdsCursors * 4 + (W / 32) * 4);
   for (uint32_t i = threadIdx.x; i < W / 32; i += kWinBlock) bits[i] = 0;
-  if (!SYM)
-    for (uint32_t i = threadIdx.x; i < W; i += kWinBlock) vals[i] = 0.0;
+  for (uint32_t i = threadIdx.x; i < W; i += kWinBlock) vals[i] = 0.0;
   __syncthreads();
   for (;;) {
     if (threadIdx.x == 0) hdr[0] = atomicAdd(counter, 1u);
@@ -899,19 +899,7 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
     int G = 1;
     while (G < 64 && (uint32_t)G < per) G <<= 1;
     const int ngroups = kWinBlock / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
-    const uint64_t off = SYM ? 0 : o.off[r];
-    // SYM: a row with a stored bitmap (bmoff[r] != ~0) gets each window's
-    // words, in 32-bit words from column left & ~127 (the layout of
-    // k_sym_dense); the words outside [lo, hi] of its first and last
-    // 128-column groups are zeroed here
-    uint32_t *sw = nullptr;
-    const uint32_t lo4 = left & ~127u;
-    if (SYM && bmoff && bmoff[r] != ~0ull) {
-      sw = bm32 + 4 * bmoff[r];
-      if (threadIdx.x < (lo - lo4) / 32) sw[threadIdx.x] = 0u;
-      const uint32_t last = (hi - lo4) / 32, nw4 = 4 * (((hi - lo4) >> 7) + 1);
-      if (threadIdx.x >= 4 && threadIdx.x - 4 < nw4 - 1 - last) sw[last + 1 + threadIdx.x - 4] = 0u;
-    }
+    const uint64_t off = o.off[r];
     for (uint32_t w = 0; w < nwin; ++w) {
       const uint64_t w0 = (uint64_t)lo + (uint64_t)w * W;
       const uint64_t w1 = min((uint64_t)hi + 1, w0 + W);
@@ -919,8 +907,7 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
         const uint32_t k = a.a_ci[a0 + jj];
         const uint32_t e = a.b_rp[k + 1];
         uint32_t c = w == 0 ? a.b_rp[k] : cur[jj];
-        double av = 0.0;
-        if (!SYM) av = (double)a.a_v[a0 + jj];
+        const double av = (double)a.a_v[a0 + jj];
         for (;;) {
           const uint32_t q = c + lig;
           const uint32_t col = q < e ? a.b_ci[q] : kEmpty;
@@ -928,7 +915,7 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
           if (valid) {
             const uint32_t x = (uint32_t)(col - w0);
             atomicOr(&bits[x >> 5], 1u << (x & 31));
-            if (!SYM) lds_add_f64(&vals[x], av * (double)a.b_v[q]);
+            lds_add_f64(&vals[x], av * (double)a.b_v[q]);
           }
           const uint32_t nv = (uint32_t)__popcll(group_ballot(valid, G));
           c += nv;
@@ -944,26 +931,18 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
       for (uint32_t i = b0; i < b1; ++i) mycnt += __popc(bits[i]);
       uint32_t wtotal;
       const uint32_t base = block_excl_sum(mycnt, hdr + 8, &wtotal);
-      if (SYM) {
-        if (sw) {
-          uint32_t *dst = sw + (uint32_t)((w0 - lo4) / 32);
-          for (uint32_t i = b0; i < b1; ++i) dst[i] = bits[i];
-        }
-        for (uint32_t i = b0; i < b1; ++i) bits[i] = 0;
-      } else {
-        uint64_t pos = off + total + base;
-        for (uint32_t i = b0; i < b1; ++i) {
-          uint32_t m = bits[i];
-          bits[i] = 0;
-          while (m) {
-            const int b = __ffs(m) - 1;
-            m &= m - 1;
-            const uint32_t x = i * 32 + b;
-            o.cols[pos] = (uint32_t)(w0 + x);
-            o.vals[pos] = (T)vals[x];
-            vals[x] = 0.0;
-            ++pos;
-          }
+      uint64_t pos = off + total + base;
+      for (uint32_t i = b0; i < b1; ++i) {
+        uint32_t m = bits[i];
+        bits[i] = 0;
+        while (m) {
+          const int b = __ffs(m) - 1;
+          m &= m - 1;
+          const uint32_t x = i * 32 + b;
+          o.cols[pos] = (uint32_t)(w0 + x);
+          o.vals[pos] = (T)vals[x];
+          vals[x] = 0.0;
+          ++pos;
         }
       }
       total += wtotal;
@@ -971,8 +950,7 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
     }
     }
     if (threadIdx.x == 0) {
-      if (SYM) nnz_sym[r] = (uint32_t)total;
-      else if (o.nnz) o.nnz[r] = (uint32_t)total;
+      if (o.nnz) o.nnz[r] = (uint32_t)total;
     }
     __syncthreads();
   }
@@ -1695,34 +1673,32 @@ size_t window_scratch_bytes(uint32_t max_a_nnz) {
   return (size_t)(grid * per * 4 + 256);
 }
 
-static size_t window_lds(uint32_t W, bool sym) {
-  return 256 + kWinLdsCursors * 4 + (W / 32) * 4 + (sym ? 0 : (size_t)W * 8);
+static size_t window_lds(uint32_t W) {
+  return 256 + kWinLdsCursors * 4 + (W / 32) * 4 + (size_t)W * 8;
 }
 
-
-template <typename T, bool SYM>
+template <typename T>
 static int window_go(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
-                     uint32_t max_a_nnz, const NumOut &o, uint32_t *nnz_sym,
-                     void *scratch, hipStream_t s, const RowSel &sel,
-                     const uint64_t *bmoff = nullptr, void *bm = nullptr) {
+                     uint32_t max_a_nnz, const NumOut &o, void *scratch, hipStream_t s,
+                     const RowSel &sel) {
   if (nrows == 0 && !sel.bounds) return OC_OK;
   const Tables &t = tables();
-  const uint32_t W = SYM ? t.win_sym_cols : t.win_num_cols;
-  const size_t lds = window_lds(W, SYM);
+  const uint32_t W = t.win_num_cols;
+  const size_t lds = window_lds(W);
   if (lds > t.lds_max_block) {
     set_error("window kernel needs %zu B of LDS, device allows %u", lds, t.lds_max_block);
     return OC_ERR_INVALID;
   }
-  OC_TRY(set_lds(k_window<T, SYM>, lds));
+  OC_TRY(set_lds(k_window<T>, lds));
   uint32_t *counter = (uint32_t *)scratch;
   uint32_t *cur = counter + 64;
   OC_TRY(zero_u32(counter, 1, s));
   const uint32_t grid = sel.bounds ? std::max<uint32_t>(1, t.num_cu)
                                   : std::min<uint32_t>(nrows, std::max<uint32_t>(1, t.num_cu));
   KOut<T> ko = kout<T>(o);
-  hipLaunchKernelGGL((k_window<T, SYM>), dim3(grid), dim3(kWinBlock), lds, s,
-                     kargs<T>(a), rows, nrows, counter, W, ko, nnz_sym, cur,
-                     max_a_nnz, sel.bounds, sel.err, bmoff, (uint32_t *)bm);
+  hipLaunchKernelGGL((k_window<T>), dim3(grid), dim3(kWinBlock), lds, s,
+                     kargs<T>(a), rows, nrows, counter, W, ko, cur, max_a_nnz, sel.bounds,
+                     sel.err);
   LAUNCH_CHECK();
   return OC_OK;
 }
@@ -1731,8 +1707,8 @@ int launch_num_window(const AccArgs &a, const uint32_t *rows, uint32_t nrows,
                       uint32_t max_a_nnz, const NumOut &o, void *scratch,
                       hipStream_t s, const RowSel &sel) {
   if (a.A.vtype == OC_F64)
-    return window_go<double, false>(a, rows, nrows, max_a_nnz, o, nullptr, scratch, s, sel);
-  return window_go<float, false>(a, rows, nrows, max_a_nnz, o, nullptr, scratch, s, sel);
+    return window_go<double>(a, rows, nrows, max_a_nnz, o, scratch, s, sel);
+  return window_go<float>(a, rows, nrows, max_a_nnz, o, scratch, s, sel);
 }
 
 // ------------------------------------------------------------ launchers

@@ -65,10 +65,6 @@ static Tables derive_tables(const int32_t *conc_in, const int32_t *block_in) {
   uint32_t w = 1u << 16;
   while (w > 1024 && (size_t)w * 8 + w / 8 + 4096 * 4 + 1024 > t.lds[6]) w >>= 1;
   t.win_num_cols = w;
-  // symbolic window: bitmap + 4096 cursors + 1 KB
-  uint32_t ws = 1u << 24;
-  while (ws > 32768 && (size_t)ws / 8 + 4096 * 4 + 1024 > t.lds[6]) ws >>= 1;
-  t.win_sym_cols = ws;
   return t;
 }
 

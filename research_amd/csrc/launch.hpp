@@ -28,7 +28,6 @@ struct Tables {
   uint32_t block[7], lds[7];
   uint32_t max_dense_sym_bin;
   uint32_t win_num_cols;   // numeric window (columns)
-  uint32_t win_sym_cols;   // symbolic window (columns)
   uint32_t lds_per_cu, num_cu, lds_max_block;
 };
 // Tables for the current device (nominal 160 KB / 256 CUs without a GPU),
