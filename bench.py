@@ -72,8 +72,14 @@ def run_batched(ra, ocean, count: int, steps: int, model_path: str | None):
         C.free()
         lat.append((time.perf_counter() - t1) * 1e3)
     plain_s = time.perf_counter() - t0
-    # batched run (whole path once over the block-diagonal operands)
-    b = ra.Batch(ra.Config())
+    # batched run (whole path once over the block-diagonal operands), with
+    # the autotuner's configuration for batches when it is there
+    tuned = os.path.join(ROOT, "research_amd", "tuned", "batched1024.json")
+    knobs = {}
+    if os.path.exists(tuned):
+        with open(tuned) as f:
+            knobs = json.load(f)
+    b = ra.Batch(ra.Config(**knobs))
     b.set_inputs(dev)
     b.run()
     ocean.lib.oc_device_synchronize()
@@ -85,6 +91,7 @@ def run_batched(ra, ocean, count: int, steps: int, model_path: str | None):
     ref_view, _, _ = b.result()
     ref_hash = ref_view.row_set_hash()
     out = {"products_count": count, "products": products, "generation_s": round(gen_s, 1),
+           "batched_config": knobs or None,
            "plain_gflops": round(2 * products / plain_s / 1e9, 3),
            "plain_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
            "plain_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
