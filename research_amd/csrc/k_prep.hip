@@ -1081,7 +1081,7 @@ int launch_assign_spare(const uint32_t *ovf_rows, uint32_t n,
 // Sums over a row list (or all rows): A entries, products and output
 // entries (from u64 offsets or a count array); out[0..3] += {rows, a_nnz,
 // products, out_nnz}.
-__global__ void k_row_sums(const uint32_t *__restrict__ rows, uint32_t n,
+__global__ __launch_bounds__(256) void k_row_sums(const uint32_t *__restrict__ rows, uint32_t n,
                            const uint32_t *__restrict__ a_rp,
                            const uint32_t *__restrict__ products,
                            const uint64_t *__restrict__ c_rp,
@@ -1100,11 +1100,21 @@ __global__ void k_row_sums(const uint32_t *__restrict__ rows, uint32_t n,
   pr = group_sum(pr, kWave);
   nz = group_sum(nz, kWave);
   k = group_sum(k, kWave);
+  // the block's four sums, then one atomic each per block (every wave
+  // adding into the same four words contended: 0.2 ms per launch)
+  __shared__ unsigned long long part[4][4];
+  const uint32_t w = threadIdx.x / kWave;
   if (lane_id() == 0) {
-    atomicAdd(&out[0], k);
-    atomicAdd(&out[1], an);
-    atomicAdd(&out[2], pr);
-    atomicAdd(&out[3], nz);
+    part[w][0] = k;
+    part[w][1] = an;
+    part[w][2] = pr;
+    part[w][3] = nz;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long s = 0;
+    for (uint32_t i = 0; i < blockDim.x / kWave; ++i) s += part[i][threadIdx.x];
+    atomicAdd(&out[threadIdx.x], s);
   }
 }
 
@@ -1112,7 +1122,7 @@ int launch_row_sums(const uint32_t *rows, uint32_t n, const uint32_t *a_rp,
                     const uint32_t *products, const uint64_t *c_rp,
                     const uint32_t *cnt, unsigned long long *out, hipStream_t s) {
   if (n == 0) return OC_OK;
-  hipLaunchKernelGGL(k_row_sums, dim3(std::min<uint32_t>(cdiv(n, 256), 1024)), dim3(256),
+  hipLaunchKernelGGL(k_row_sums, dim3(std::min<uint32_t>(cdiv(n, 256), 256)), dim3(256),
                      0, s, rows, n, a_rp, products, c_rp, cnt, out);
   LAUNCH_CHECK();
   return OC_OK;
