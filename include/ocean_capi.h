@@ -433,7 +433,7 @@ int oc_mlp_class_thresholds(uint32_t *thr);
 int oc_mlp_features(const oc_csr *dA, const oc_csr *dB, float *feat, void *stream);
 /* Class labels (device u8) of exact per-row counts (device u32). */
 int oc_mlp_labels(const uint32_t *nnz, uint64_t n, uint8_t *labels, void *stream);
-/* Logits (device out, n x 8 floats). */
+/* Logits (device out, n x 8 floats); feat 16-byte aligned. */
 int oc_mlp_forward(const oc_mlp *m, const float *feat, uint64_t n, float *logits,
                    void *stream);
 /* Gradients of the mean cross-entropy of one minibatch (n <= 65536) into

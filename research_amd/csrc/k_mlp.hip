@@ -149,6 +149,23 @@ __device__ void stage_x(MlpSmem &s, const float *__restrict__ feat, uint64_t row
   }
 }
 
+// Features of one wave's 16 rows [g0, g0+16) into s.X rows r0..r0+15,
+// columns 0..15 (one float4 per lane; rows beyond n are zero).  The padding
+// columns 16..31 are zeroed once by the caller.
+__device__ void stage_x_wave(MlpSmem &s, const float *__restrict__ feat, uint64_t g0,
+                             uint64_t n, int r0) {
+  static_assert(kF == 16, "one float4 per lane covers 16 rows of 16 features");
+  const int l = lane_id(), r = l >> 2, f = (l & 3) * 4;
+  const uint64_t g = g0 + r;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (g < n) v = *reinterpret_cast<const float4 *>(feat + g * kF + f);
+  uint16_t *x = s.X + (r0 + r) * kS32 + f;
+  x[0] = f2bf(v.x);
+  x[1] = f2bf(v.y);
+  x[2] = f2bf(v.z);
+  x[3] = f2bf(v.w);
+}
+
 // Forward pass of the wave's 16 rows (block rows r0..r0+15): H1, H2 into
 // LDS; returns the logits accumulator (lane: class l%16, rows
 // r0 + (l/16)*4 + r) with b3 added.
@@ -211,6 +228,9 @@ struct EstArgs {
   uint32_t *est;
 };
 
+// Persistent: each workgroup stages the weights once and strides over the
+// 64-row groups; within a group every wave loads and runs its own 16 rows,
+// so the loop needs no workgroup barrier.
 template <int MODE>
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_forward(
     const uint16_t *__restrict__ wbf, const float *__restrict__ master,
@@ -218,11 +238,16 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_forward(
   extern __shared__ __align__(16) uint8_t smem_raw[];
   MlpSmem &s = *reinterpret_cast<MlpSmem *>(smem_raw);
   stage_weights(s, wbf, master);
-  const uint64_t row0 = (uint64_t)blockIdx.x * kRowsPerBlock;
-  stage_x(s, feat, row0, n);
+  for (int i = threadIdx.x; i < kRowsPerBlock * 16; i += blockDim.x)
+    s.X[(i / 16) * kS32 + 16 + i % 16] = 0;
   __syncthreads();
   const int w = threadIdx.x / kWave, l = lane_id();
   const int r0 = w * 16, col = l & 15, rb = (l >> 4) * 4;
+  const uint64_t ngroups = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+  for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+  const uint64_t row0 = grp * kRowsPerBlock;
+  stage_x_wave(s, feat, row0 + r0, n, r0);
+  wave_sync();
   const f4 z = forward_wave(s, r0);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -250,6 +275,8 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_forward(
         ea.est[g] = min(e, ub);
       }
     }
+  }
+  wave_sync();  // this wave's X, H1 and H2 rows are reused by its next group
   }
 }
 
@@ -475,6 +502,21 @@ static int set_lds_mlp(K kernel, size_t bytes) {
   return OC_OK;
 }
 
+// Workgroups of the persistent forward pass: every resident one, or one per
+// 64-row group when there are fewer groups.
+template <typename K>
+static uint32_t forward_grid(K kernel, size_t lds, uint64_t n) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kMlpBlock, lds) !=
+          hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  const uint64_t full = (uint64_t)std::max<uint32_t>(1, tables().num_cu) * (uint32_t)per_cu;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, (n + kRowsPerBlock - 1) / kRowsPerBlock));
+}
+
 static uint32_t grid_1d(uint64_t n, uint32_t bs) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + bs - 1) / bs, 1u << 16));
 }
@@ -515,6 +557,7 @@ namespace oc {
 int launch_mlp_estimates(const oc_mlp *m, const float *feat, uint64_t n, const RowInfo &ri,
                          double q, uint32_t *est, hipStream_t s) {
   if (n == 0) return OC_OK;
+  OC_REQUIRE((uintptr_t)feat % 16 == 0, "predictor features must be 16-byte aligned");
   EstArgs ea;
   ea.products = ri.products;
   ea.max_b = ri.max_b;
@@ -525,9 +568,8 @@ int launch_mlp_estimates(const oc_mlp *m, const float *feat, uint64_t n, const R
   ea.est = est;
   const size_t lds = sizeof(MlpSmem);
   OC_TRY(set_lds_mlp(k_mlp_forward<1>, lds));
-  const uint64_t nblk = (n + kRowsPerBlock - 1) / kRowsPerBlock;
-  OC_REQUIRE(nblk < (1ull << 31), "too many rows for the predictor");
-  hipLaunchKernelGGL((k_mlp_forward<1>), dim3((uint32_t)nblk), dim3(kMlpBlock), lds, s,
+  const uint32_t nblk = forward_grid(k_mlp_forward<1>, lds, n);
+  hipLaunchKernelGGL((k_mlp_forward<1>), dim3(nblk), dim3(kMlpBlock), lds, s,
                      m->wbf, m->master, feat, n, nullptr, ea);
   LAUNCH_CHECK();
   return OC_OK;
@@ -699,14 +741,15 @@ extern "C" int oc_mlp_forward(const oc_mlp *m, const float *feat, uint64_t n, fl
                               void *stream) {
   OC_REQUIRE(m && ((feat && logits) || n == 0), "oc_mlp_forward: null argument");
   if (n == 0) return OC_OK;
+  OC_REQUIRE((uintptr_t)feat % 16 == 0, "oc_mlp_forward: features must be 16-byte aligned");
+  OC_TRY(tables_init());
   hipStream_t s = (hipStream_t)stream;
-  const uint64_t nblk = (n + kRowsPerBlock - 1) / kRowsPerBlock;
-  OC_REQUIRE(nblk < (1ull << 31), "too many rows");
   const size_t lds = sizeof(MlpSmem);
   OC_TRY(set_lds_mlp(k_mlp_forward<0>, lds));
+  const uint32_t nblk = forward_grid(k_mlp_forward<0>, lds, n);
   EstArgs ea;
   std::memset(&ea, 0, sizeof(ea));
-  hipLaunchKernelGGL((k_mlp_forward<0>), dim3((uint32_t)nblk), dim3(kMlpBlock), lds, s,
+  hipLaunchKernelGGL((k_mlp_forward<0>), dim3(nblk), dim3(kMlpBlock), lds, s,
                      m->wbf, m->master, feat, n, logits, ea);
   LAUNCH_CHECK();
   HIP_TRY(hipStreamSynchronize(s));

@@ -48,7 +48,7 @@ def _data(n, seed):
     return X, y
 
 
-@pytest.mark.parametrize("n", [1, 64, 1000, 65536])
+@pytest.mark.parametrize("n", [1, 64, 1000, 65536, 200001])
 def test_forward_matches_oracle(gpu, n):
     log(f"case: mlp forward n={n}")
     p = _rand_params(5)
