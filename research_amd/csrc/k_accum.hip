@@ -960,16 +960,23 @@ __global__ __launch_bounds__(kWinBlock) void k_window(
 // Column tiles of W = 2^logW columns.  ts[k * T1 + t] is the index of the
 // first entry of B row k whose column is >= t * W (T1 = tiles + 1), so the
 // part of any B row inside any tile is known without walking the row.
+// A wave takes 64 / P rows at once, P = 2^logP lanes per row (T1 rounded up
+// to a power of two, at most 64), so narrow tables keep every lane busy.
 __global__ __launch_bounds__(256) void k_tile_starts(const uint32_t *__restrict__ b_rp,
                                                      const uint32_t *__restrict__ b_ci,
                                                      uint32_t rows, uint32_t logW,
                                                      uint32_t T1, const uint32_t *__restrict__ base,
-                                                     uint32_t *__restrict__ ts) {
-  const int lane = lane_id();
-  for (uint32_t k = blockIdx.x * 4 + threadIdx.x / kWave; k < rows; k += gridDim.x * 4) {
+                                                     uint32_t *__restrict__ ts, uint32_t logP) {
+  const uint32_t lane = (uint32_t)lane_id(), P = 1u << logP, sub = lane >> logP;
+  const uint32_t per_wave = kWave >> logP;
+  const uint32_t waves = gridDim.x * 4;
+  for (uint32_t k0 = (blockIdx.x * 4 + threadIdx.x / kWave) * per_wave; k0 < rows;
+       k0 += waves * per_wave) {
+    const uint32_t k = k0 + sub;
+    if (k >= rows) continue;
     const uint32_t s = b_rp[k], e = b_rp[k + 1];
     const uint64_t b0 = base ? base[k] : 0;  // tiles counted from the row's block
-    for (uint32_t t = lane; t < T1; t += kWave) {
+    for (uint32_t t = lane & (P - 1); t < T1; t += P) {
       const uint64_t bound = b0 + ((uint64_t)t << logW);
       uint32_t lo = s, hi = e;
       while (lo < hi) {
@@ -1627,8 +1634,12 @@ int tile_plan(uint64_t b_rows, uint64_t b_cols, uint32_t log2_cols, uint32_t bs,
 int launch_tile_starts(const DevCsr &B, uint32_t logW, uint32_t T1, uint32_t *ts,
                        hipStream_t s, const uint32_t *base) {
   if (B.rows == 0) return OC_OK;
-  hipLaunchKernelGGL(k_tile_starts, dim3(std::min<uint32_t>(cdiv(B.rows, 4), 1u << 20)),
-                     dim3(256), 0, s, B.rp, B.ci, (uint32_t)B.rows, logW, T1, base, ts);
+  uint32_t logP = 0;
+  while (logP < 6 && (1u << logP) < T1) ++logP;
+  const uint32_t rows_per_wg = 4u * (64u >> logP);
+  hipLaunchKernelGGL(k_tile_starts,
+                     dim3(std::min<uint32_t>(cdiv(B.rows, rows_per_wg), 1u << 20)), dim3(256),
+                     0, s, B.rp, B.ci, (uint32_t)B.rows, logW, T1, base, ts, logP);
   LAUNCH_CHECK();
   return OC_OK;
 }
