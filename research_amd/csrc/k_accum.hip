@@ -248,6 +248,25 @@ bad:
   return OC_ERR_INVALID;
 }
 
+// =========================================================== row scalars
+// The per-row scalars every row kernel needs, loaded together when the row
+// starts: independent loads share one memory round trip instead of each
+// waiting behind the previous barrier.
+struct RowHead {
+  uint32_t a0, a1, prod, maxb, left, right;
+};
+template <typename KA>
+__device__ __forceinline__ RowHead row_head(const KA &a, uint32_t r) {
+  RowHead h;
+  h.a0 = a.a_rp[r];
+  h.a1 = a.a_rp[r + 1];
+  h.prod = a.products[r];
+  h.maxb = a.max_b[r];
+  h.left = a.left[r];
+  h.right = a.right[r];
+  return h;
+}
+
 // ============================================================ heavy rows
 // A B row far longer than a lane group's share of the row's products (an
 // R-MAT hub) keeps its group busy long after the others finish.  The
@@ -257,13 +276,13 @@ bad:
 // groups take every row); the listing pass, which ends with a workgroup
 // barrier, is skipped when the row's longest B row is not heavy.
 template <typename KA>
-__device__ __forceinline__ uint32_t list_heavy(const KA &a, uint32_t r, uint32_t a0,
+__device__ __forceinline__ uint32_t list_heavy(const KA &a, const RowHead &rh, uint32_t a0,
                                                uint32_t a1, int G, int gid, int ngroups,
                                                int lig, uint32_t *count, uint32_t *list,
                                                uint32_t cap, uint32_t *heavy) {
-  const uint32_t share = (a.products[r] + ngroups - 1) / ngroups;
+  const uint32_t share = (rh.prod + ngroups - 1) / ngroups;
   *heavy = max(8u * (uint32_t)G, 2u * share);
-  if (a.max_b[r] <= *heavy) return 0u;  // no heavy row: no listing pass
+  if (rh.maxb <= *heavy) return 0u;  // no heavy row: no listing pass
   for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
     const uint32_t k = a.a_ci[j];
     if (lig == 0 && a.b_rp[k + 1] - a.b_rp[k] > *heavy) {
@@ -327,12 +346,13 @@ __device__ __forceinline__ void sym_hash_row(KArgs<float> a,
   uint32_t *hdr = lds_u32;
   uint32_t *tab = lds_u32 + 64;
   const bool track = limit < S;
+  const RowHead rh = row_head(a, r);
   // With full-size tables (no load limit), a row whose column span fits the
   // table's S words as a bitmap is counted with ORs and popcounts instead
   // of probing; both give the exact count.  Shrunken tables keep hashing,
   // so the rows that outgrow them are listed as before.
-  const uint32_t blo = a.left[r] & ~31u, bhi = a.right[r];
-  const uint32_t bw = bhi >= a.left[r] ? ((bhi - blo) >> 5) + 1 : 0u;
+  const uint32_t blo = rh.left & ~31u, bhi = rh.right;
+  const uint32_t bw = bhi >= rh.left ? ((bhi - blo) >> 5) + 1 : 0u;
   const bool use_bm = !track && bw && bw <= S;
   if (use_bm) {
     for (uint32_t i = threadIdx.x; i < bw; i += BS) tab[i] = 0u;
@@ -341,8 +361,8 @@ __device__ __forceinline__ void sym_hash_row(KArgs<float> a,
   }
   if (threadIdx.x < 4) hdr[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
-  const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
+  const uint32_t a0 = rh.a0, a1 = rh.a1;
+  const int G = pick_group(a1 - a0, rh.prod, rh.maxb, BS);
   const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
   if (use_bm) {
     auto mark = [&](uint32_t col) {
@@ -351,7 +371,7 @@ __device__ __forceinline__ void sym_hash_row(KArgs<float> a,
     };
     uint32_t heavy;
     const uint32_t nheavy =
-        list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
+        list_heavy(a, rh, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
     for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
       const uint32_t k = a.a_ci[j];
       const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
@@ -411,7 +431,7 @@ __device__ __forceinline__ void sym_hash_row(KArgs<float> a,
   };
   uint32_t heavy;
   const uint32_t nheavy =
-      list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
+      list_heavy(a, rh, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
   for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
     if (track && lds_load(&hdr[0]) > limit) {  // past the load limit
       ovf = true;
@@ -472,9 +492,10 @@ __device__ __forceinline__ void sym_dense_row(KArgs<float> a,
   extern __shared__ uint32_t lds_u32[];
   uint32_t *hdr = lds_u32;
   uint32_t *bits = lds_u32 + 64;
-  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
-  const uint32_t lo = a.left[r] & ~127u, hi = a.right[r];
-  if (a1 == a0 || a.products[r] == 0 || hi < a.left[r]) {
+  const RowHead rh = row_head(a, r);
+  const uint32_t a0 = rh.a0, a1 = rh.a1;
+  const uint32_t lo = rh.left & ~127u, hi = rh.right;
+  if (a1 == a0 || rh.prod == 0 || hi < rh.left) {
     if (threadIdx.x == 0) nnz[r] = 0;
     return;
   }
@@ -483,11 +504,11 @@ __device__ __forceinline__ void sym_dense_row(KArgs<float> a,
   uint4 *bits4 = reinterpret_cast<uint4 *>(bits);
   if (threadIdx.x == 0) hdr[0] = hdr[4] = 0;
   __syncthreads();
-  const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
+  const int G = pick_group(a1 - a0, rh.prod, rh.maxb, BS);
   const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
   uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
   const uint32_t nheavy =
-      list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[4], hdr + 32, kDenseHeavy, &heavy);
+      list_heavy(a, rh, a0, a1, G, gid, ngroups, lig, &hdr[4], hdr + 32, kDenseHeavy, &heavy);
   for (uint32_t w = 0; w < nwin; ++w) {
     // window [w0, w1] (the whole span without WIN); the bitmap is cleared
     // and counted 128 bits at a time (the launch allocates whole 16-byte
@@ -585,17 +606,20 @@ __device__ __forceinline__ void num_hash_row(KArgs<T> a,
   uint16_t *ci = reinterpret_cast<uint16_t *>(smem + L.off_ci);
   uint16_t *cnt = reinterpret_cast<uint16_t *>(smem + L.off_cnt);
   const uint32_t S = L.S;
+  const RowHead rh = row_head(a, r);
+  const uint64_t off = o.off[r];
+  const uint32_t cap = EST ? o.cap[r] : 0u;
   for (uint32_t i = threadIdx.x; i < S; i += BS) {
     keys[i] = kEmpty;
     vals[i] = 0.0;
   }
   if (threadIdx.x < 4) hdr[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
-  const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
+  const uint32_t a0 = rh.a0, a1 = rh.a1;
+  const int G = pick_group(a1 - a0, rh.prod, rh.maxb, BS);
   const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
   uint32_t limit = S;
-  if (EST) limit = min(L.nmax - 1, o.cap[r]);
+  if (EST) limit = min(L.nmax - 1, cap);
   bool ovf = false;
   // linear probing from `slot` (`probes` slots already tried), each probe
   // a compare-and-swap that either claims an empty slot or finds the key;
@@ -635,7 +659,7 @@ __device__ __forceinline__ void num_hash_row(KArgs<T> a,
   };
   uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
   const uint32_t nheavy =
-      list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
+      list_heavy(a, rh, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
   for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
     if (EST && lds_load(&hdr[0]) > limit) {  // more keys than the capacity
       ovf = true;
@@ -659,12 +683,12 @@ __device__ __forceinline__ void num_hash_row(KArgs<T> a,
   }
   if (ovf) hdr[1] = 1;
   __syncthreads();
-  const uint32_t lo = a.left[r];
+  const uint32_t lo = rh.left;
   // Output order by rank, no sort: when a bitmap of the row's column span
   // and one prefix per word (8 bytes per 32 columns) fit in the sort's
   // scratch, each key's place in the output is the popcount of the set
   // bits before it.  Wider rows compact the occupied slots and sort.
-  const uint32_t spanw = a.right[r] >= lo ? (a.right[r] - lo) / 32 + 1 : 0u;
+  const uint32_t spanw = rh.right >= lo ? (rh.right - lo) / 32 + 1 : 0u;
   if (!hdr[1] && spanw && 8ull * spanw <= L.bytes - L.off_ck) {
     uint32_t *bits = reinterpret_cast<uint32_t *>(smem + L.off_ck);
     uint32_t *pref = bits + spanw;
@@ -689,12 +713,11 @@ __device__ __forceinline__ void num_hash_row(KArgs<T> a,
       pref[w] = run;
       run += __popc(bits[w]);
     }
-    if (n > L.nmax || (EST && n > o.cap[r])) {  // as the sorting path
+    if (n > L.nmax || (EST && n > cap)) {  // as the sorting path
       if (threadIdx.x == 0) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
       return;
     }
     __syncthreads();
-    const uint64_t off = o.off[r];
     for (uint32_t i = threadIdx.x; i < S; i += BS) {
       const uint32_t kk = keys[i];
       if (kk != kEmpty) {
@@ -723,7 +746,7 @@ __device__ __forceinline__ void num_hash_row(KArgs<T> a,
   }
   __syncthreads();
   const uint32_t n = hdr[2];
-  const bool overflow = hdr[1] || n > L.nmax || (EST && n > o.cap[r]);
+  const bool overflow = hdr[1] || n > L.nmax || (EST && n > cap);
   if (overflow) {
     if (threadIdx.x == 0) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
     return;
@@ -731,12 +754,11 @@ __device__ __forceinline__ void num_hash_row(KArgs<T> a,
   uint32_t *k1 = keys;
   uint16_t *i1 = reinterpret_cast<uint16_t *>(keys + L.nmax);
   // every output column lies in [left, right]; an empty row has n == 0
-  const uint32_t span = n ? a.right[r] - lo : 0u;
+  const uint32_t span = n ? rh.right - lo : 0u;
   const int kb = min(key_bits, span ? 32 - __clz(span) : 0);
   const int parity = block_radix_sort(ck, ci, k1, i1, cnt, dsum, (int)n, kb);
   const uint32_t *sk = parity ? k1 : ck;
   const uint16_t *si = parity ? i1 : ci;
-  const uint64_t off = o.off[r];
   for (uint32_t i = threadIdx.x; i < n; i += BS) {
     o.cols[off + i] = sk[i] + lo;
     o.vals[off + i] = (T)vals[si[i]];
@@ -769,9 +791,12 @@ __device__ __forceinline__ void num_dense_row(KArgs<T> a,
                                                   KOut<T> o) {
   extern __shared__ __align__(16) uint8_t smem[];
   uint32_t *hdr = reinterpret_cast<uint32_t *>(smem);
-  const uint32_t a0 = a.a_rp[r], a1 = a.a_rp[r + 1];
-  const uint32_t lo = a.left[r] & ~31u, hi = a.right[r];
-  if (a1 == a0 || a.products[r] == 0 || hi < a.left[r]) {
+  const RowHead rh = row_head(a, r);
+  const uint64_t off = o.off[r];
+  const uint32_t cap = EST ? o.cap[r] : 0u;
+  const uint32_t a0 = rh.a0, a1 = rh.a1;
+  const uint32_t lo = rh.left & ~31u, hi = rh.right;
+  if (a1 == a0 || rh.prod == 0 || hi < rh.left) {
     if (threadIdx.x == 0 && o.nnz) o.nnz[r] = 0;
     return;
   }
@@ -782,7 +807,7 @@ __device__ __forceinline__ void num_dense_row(KArgs<T> a,
   for (uint32_t i = threadIdx.x; i < nwords; i += BS) bits[i] = 0;
   if (threadIdx.x == 0) hdr[4] = 0;
   __syncthreads();
-  const int G = pick_group(a1 - a0, a.products[r], a.max_b[r], BS);
+  const int G = pick_group(a1 - a0, rh.prod, rh.maxb, BS);
   const int ngroups = BS / G, gid = threadIdx.x / G, lig = threadIdx.x % G;
   auto add = [&](uint32_t col, double pv) {
     const uint32_t c = col - lo;
@@ -791,7 +816,7 @@ __device__ __forceinline__ void num_dense_row(KArgs<T> a,
   };
   uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
   const uint32_t nheavy =
-      list_heavy(a, r, a0, a1, G, gid, ngroups, lig, &hdr[4], hdr + 32, kDenseHeavy, &heavy);
+      list_heavy(a, rh, a0, a1, G, gid, ngroups, lig, &hdr[4], hdr + 32, kDenseHeavy, &heavy);
   for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
     const uint32_t k = a.a_ci[j];
     const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
@@ -811,11 +836,11 @@ __device__ __forceinline__ void num_dense_row(KArgs<T> a,
   for (uint32_t w = w0; w < w1; ++w) mycnt += __popc(bits[w]);
   uint32_t total;
   const uint32_t base = block_excl_sum(mycnt, hdr + 8, &total);
-  if (EST && total > o.cap[r]) {
+  if (EST && total > cap) {
     if (threadIdx.x == 0) o.ovf_rows[atomicAdd(o.ovf_count, 1u)] = r;
     return;
   }
-  uint64_t pos = o.off[r] + base;
+  uint64_t pos = off + base;
   for (uint32_t w = w0; w < w1; ++w) {
     uint32_t m = bits[w];
     while (m) {
