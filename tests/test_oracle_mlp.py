@@ -19,7 +19,7 @@ def test_bf16_round_nearest_even():
         0.0: 0.0,
     }
     for x, want in cases.items():
-        got = float(mlp.bf16(np.float32(x))[()])
+        got = float(mlp.bf16(np.float32(x)).reshape(-1)[0])
         assert got == pytest.approx(want, rel=1e-7, abs=0), x
     # every bf16 value is a fixed point
     u = np.arange(0, 1 << 16, 97, dtype=np.uint32) << 16
