@@ -353,15 +353,23 @@ __device__ __forceinline__ void claim_rows(const uint32_t *__restrict__ bounds, 
   // LDS off its 16-byte alignment (k_sym_dense 32.5 -> 37.2 ms), and 16
   // bytes took a workgroup per CU from bins sized to the CU's LDS
   // (k_sym_hash 4.6 -> 6.4 ms on R-MAT 2^20).
+  // Rows per claim: kClaimRows, fewer (down to one) when the bin has
+  // fewer than 16 claims per workgroup, so that a bin of few, heavy rows
+  // spreads them over the grid instead of queueing them on a handful of
+  // workgroups, and leaves a short tail (a 38,424-row panel of R-MAT 2^21
+  // degree 32: windowed symbolic rows 33.1 -> 9.6 ms; the 48 panels of
+  // that product 3.09 -> 2.17 s).
   extern __shared__ uint32_t s_claim[];
   const uint32_t b0 = bounds[0], b1 = bounds[1];
+  const uint32_t n = b1 - min(b0, b1);
+  const uint32_t chunk = max(1u, min(kClaimRows, n / (16u * gridDim.x)));
   for (;;) {
-    if (threadIdx.x == 0) s_claim[0] = atomicAdd(claim, kClaimRows);
+    if (threadIdx.x == 0) s_claim[0] = atomicAdd(claim, chunk);
     __syncthreads();
     const uint32_t c = s_claim[0];
     __syncthreads();
-    if (c >= b1 - min(b0, b1)) break;
-    const uint32_t e = min(b1, b0 + c + kClaimRows);
+    if (c >= n) break;
+    const uint32_t e = min(b1, b0 + c + chunk);
     for (uint32_t i = b0 + c; i < e; ++i) {
       row(i);
       __syncthreads();  // the next row re-initialises the LDS
@@ -1823,11 +1831,11 @@ static uint32_t persistent_grid(K kernel, uint32_t bs, size_t lds) {
 }
 
 // The grid of a persistent launch over a bin: every resident workgroup, or
-// one per kClaimRows rows when the host knows the bin's size (nrows > 0).
+// at most one per row when the host knows the bin's size (nrows > 0).
 template <typename K>
 static uint32_t claim_grid(K kernel, uint32_t bs, size_t lds, uint32_t nrows) {
   const uint32_t full = persistent_grid(kernel, bs, lds);
-  return nrows ? std::min<uint32_t>(full, cdiv(nrows, kClaimRows)) : full;
+  return nrows ? std::min<uint32_t>(full, nrows) : full;
 }
 
 // A bin launch whose host knows the bin (sel.first, nrows) runs one
