@@ -290,6 +290,35 @@ bad:
   return OC_ERR_INVALID;
 }
 
+// ====================================================== A entries of a row
+// The entries j = a0 + gid, a0 + gid + ngroups, ... of a row with their B
+// row bounds, software-pipelined: the next entry's bounds and the column of
+// the entry after it are loaded while f sweeps entry j, so a group waits
+// about one memory latency per entry instead of three (A column, B row
+// bounds, B entries).  f(j, bs, be) returns false to stop.  (Dense
+// symbolic rows: R-MAT 2^20 25.8 -> 25.1 ms, a 512-product batch 11.2 ->
+// 10.7 ms; the other row kernels unchanged or slightly faster.)
+template <typename KA, typename F>
+__device__ __forceinline__ void for_entries(const KA &a, uint32_t a0, uint32_t a1, uint32_t gid,
+                                            uint32_t ngroups, F &&f) {
+  uint32_t j = a0 + gid;
+  if (j >= a1) return;
+  const uint32_t k = a.a_ci[j];
+  uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
+  uint32_t kn = j + ngroups < a1 ? a.a_ci[j + ngroups] : 0u;
+  for (;;) {
+    const uint32_t cj = j, cbs = bs, cbe = be;
+    j += ngroups;
+    const bool more = j < a1;
+    if (more) {
+      bs = a.b_rp[kn];
+      be = a.b_rp[kn + 1];
+      if (j + ngroups < a1) kn = a.a_ci[j + ngroups];
+    }
+    if (!f(cj, cbs, cbe) || !more) break;
+  }
+}
+
 // =========================================================== row scalars
 // The per-row scalars every row kernel needs, loaded together when the row
 // starts: independent loads share one memory round trip instead of each
@@ -422,12 +451,12 @@ __device__ __forceinline__ void sym_hash_row(KArgs<float> a,
     uint32_t heavy;
     const uint32_t nheavy =
         list_heavy(a, rh, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
-    for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
-      const uint32_t k = a.a_ci[j];
-      const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
-      if (nheavy && be - bs > heavy) continue;  // listed: swept below
-      seg_cols<8, false>(bs, be, lig, G, a.b_ci, mark);
-    }
+    for_entries(a, a0, a1, (uint32_t)gid, (uint32_t)ngroups,
+                [&](uint32_t, uint32_t bs, uint32_t be) {
+                  if (!(nheavy && be - bs > heavy))  // listed: swept below
+                    seg_cols<8, false>(bs, be, lig, G, a.b_ci, mark);
+                  return true;
+                });
     for (uint32_t h = 0; h < nheavy; ++h) {
       const uint32_t k = a.a_ci[hdr[8 + h]];
       seg_cols<8, false>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, mark);
@@ -482,16 +511,16 @@ __device__ __forceinline__ void sym_hash_row(KArgs<float> a,
   uint32_t heavy;
   const uint32_t nheavy =
       list_heavy(a, rh, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
-  for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
-    if (track && lds_load(&hdr[0]) > limit) {  // past the load limit
-      ovf = true;
-      break;
-    }
-    const uint32_t k = a.a_ci[j];
-    const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
-    if (nheavy && be - bs > heavy) continue;  // listed: swept below
-    seg_cols_batch<8>(bs, be, lig, G, a.b_ci, insert8, insert);
-  }
+  for_entries(a, a0, a1, (uint32_t)gid, (uint32_t)ngroups,
+              [&](uint32_t, uint32_t bs, uint32_t be) {
+                if (track && lds_load(&hdr[0]) > limit) {  // past the load limit
+                  ovf = true;
+                  return false;
+                }
+                if (!(nheavy && be - bs > heavy))  // listed: swept below
+                  seg_cols_batch<8>(bs, be, lig, G, a.b_ci, insert8, insert);
+                return !ovf;
+              });
   __syncthreads();
   for (uint32_t h = 0; h < nheavy && !ovf; ++h) {
     if (track && lds_load(&hdr[0]) > limit) {
@@ -572,12 +601,12 @@ __device__ __forceinline__ void sym_dense_row(KArgs<float> a,
       const uint32_t c = col - w0;
       if (!WIN || (col >= w0 && col <= w1)) atomicOr(&bits[c >> 5], 1u << (c & 31));
     };
-    for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
-      const uint32_t k = a.a_ci[j];
-      const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
-      if (nheavy && be - bs > heavy) continue;
-      seg_cols<8, BS == 1024>(bs, be, lig, G, a.b_ci, mark);
-    }
+    for_entries(a, a0, a1, (uint32_t)gid, (uint32_t)ngroups,
+                [&](uint32_t, uint32_t bs, uint32_t be) {
+                  if (!(nheavy && be - bs > heavy))
+                    seg_cols<8, BS == 1024>(bs, be, lig, G, a.b_ci, mark);
+                  return true;
+                });
     for (uint32_t h = 0; h < nheavy; ++h) {
       const uint32_t k = a.a_ci[hdr[32 + h]];
       seg_cols<8, BS == 1024>(a.b_rp[k], a.b_rp[k + 1], (int)threadIdx.x, BS, a.b_ci, mark);
@@ -710,16 +739,17 @@ __device__ __forceinline__ void num_hash_row(KArgs<T> a,
   uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
   const uint32_t nheavy =
       list_heavy(a, rh, a0, a1, G, gid, ngroups, lig, &hdr[3], hdr + 8, kSymHeavy, &heavy);
-  for (uint32_t j = a0 + gid; j < a1 && !ovf; j += ngroups) {
-    if (EST && lds_load(&hdr[0]) > limit) {  // more keys than the capacity
-      ovf = true;
-      break;
-    }
-    const uint32_t k = a.a_ci[j];
-    const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
-    if (nheavy && be - bs > heavy) continue;  // listed: swept below
-    seg_vals_batch<4>(bs, be, lig, G, (double)a.a_v[j], a.b_ci, a.b_v, insert4, insert);
-  }
+  for_entries(a, a0, a1, (uint32_t)gid, (uint32_t)ngroups,
+              [&](uint32_t j, uint32_t bs, uint32_t be) {
+                if (EST && lds_load(&hdr[0]) > limit) {  // more keys than the capacity
+                  ovf = true;
+                  return false;
+                }
+                if (!(nheavy && be - bs > heavy))  // listed: swept below
+                  seg_vals_batch<4>(bs, be, lig, G, (double)a.a_v[j], a.b_ci, a.b_v, insert4,
+                                    insert);
+                return !ovf;
+              });
   __syncthreads();
   for (uint32_t h = 0; h < nheavy && !ovf; ++h) {
     if (EST && lds_load(&hdr[0]) > limit) {
@@ -867,12 +897,12 @@ __device__ __forceinline__ void num_dense_row(KArgs<T> a,
   uint32_t heavy;  // heavy B rows listed and swept by the whole workgroup
   const uint32_t nheavy =
       list_heavy(a, rh, a0, a1, G, gid, ngroups, lig, &hdr[4], hdr + 32, kDenseHeavy, &heavy);
-  for (uint32_t j = a0 + gid; j < a1; j += ngroups) {
-    const uint32_t k = a.a_ci[j];
-    const uint32_t bs = a.b_rp[k], be = a.b_rp[k + 1];
-    if (nheavy && be - bs > heavy) continue;
-    seg_vals<4>(bs, be, lig, G, (double)a.a_v[j], a.b_ci, a.b_v, add);
-  }
+  for_entries(a, a0, a1, (uint32_t)gid, (uint32_t)ngroups,
+              [&](uint32_t j, uint32_t bs, uint32_t be) {
+                if (!(nheavy && be - bs > heavy))
+                  seg_vals<4>(bs, be, lig, G, (double)a.a_v[j], a.b_ci, a.b_v, add);
+                return true;
+              });
   for (uint32_t h = 0; h < nheavy; ++h) {
     const uint32_t j = hdr[32 + h];
     const uint32_t k = a.a_ci[j];
