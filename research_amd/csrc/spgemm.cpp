@@ -1762,9 +1762,14 @@ int spgemm_impl(oc_spgemm *h, const oc_csr *dA, const oc_csr *dB, oc_csr *dC,
       if (rc == OC_OK && learned && (use_est || h->force_workflow == 1)) {
         h->learned_run = true;
         h->d.ana2_type = 1;  // predicted capacities
+      } else if (rc == OC_OK && learned) {
+        // the predictor picked exact counts: the precise workflow runs with
+        // full-size symbolic tables, without the HyperLogLog sketches and
+        // sampling, which would only choose the workflow again and maybe
+        // shrink those tables
+        h->d.ana2_type = 0;
       } else if (rc == OC_OK && h->d.ana1_type == 1) {
         rc = r.sampling();
-        if (learned) h->d.ana2_type = 0;  // the predictor picked exact counts
       }
       if (rc == OC_OK && h->force_workflow >= 0 && h->d.ana1_type == 1)
         h->d.ana2_type = h->force_workflow;
