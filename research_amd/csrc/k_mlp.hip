@@ -245,38 +245,38 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_forward(
   const int r0 = w * 16, col = l & 15, rb = (l >> 4) * 4;
   const uint64_t ngroups = (n + kRowsPerBlock - 1) / kRowsPerBlock;
   for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-  const uint64_t row0 = grp * kRowsPerBlock;
-  stage_x_wave(s, feat, row0 + r0, n, r0);
-  wave_sync();
-  const f4 z = forward_wave(s, r0);
+    const uint64_t row0 = grp * kRowsPerBlock;
+    stage_x_wave(s, feat, row0 + r0, n, r0);
+    wave_sync();
+    const f4 z = forward_wave(s, r0);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const uint64_t g = row0 + r0 + rb + r;
-    if (MODE == 0) {
-      if (col < kC && g < n) logits[g * kC + col] = z[r];
-    } else {
-      float p, m, sum;
-      softmax8(z[r], p, m, sum);
-      // inclusive prefix of p over the 8 class lanes of the row
-      float cum = p;
+    for (int r = 0; r < 4; ++r) {
+      const uint64_t g = row0 + r0 + rb + r;
+      if (MODE == 0) {
+        if (col < kC && g < n) logits[g * kC + col] = z[r];
+      } else {
+        float p, m, sum;
+        softmax8(z[r], p, m, sum);
+        // inclusive prefix of p over the 8 class lanes of the row
+        float cum = p;
 #pragma unroll
-      for (int d = 1; d < 8; d <<= 1) {
-        const float o = __shfl_up(cum, d, 8);
-        if ((col & 7) >= d) cum += o;
-      }
-      const uint64_t below = __ballot(col < kC && cum < ea.q);
-      const int cls = min(kC - 1, (int)__popcll((below >> ((l >> 4) * 16)) & 0xFFull));
-      if (col == 0 && g < n) {
-        const uint32_t P = ea.products[g];
-        const uint32_t span = ea.right[g] >= ea.left[g] ? ea.right[g] - ea.left[g] + 1 : 0u;
-        const uint32_t ub = min(P, span);
-        uint32_t e = cls < kC - 1 ? ea.thr[cls] : ub;
-        e = max(e, ea.max_b[g]);
-        ea.est[g] = min(e, ub);
+        for (int d = 1; d < 8; d <<= 1) {
+          const float o = __shfl_up(cum, d, 8);
+          if ((col & 7) >= d) cum += o;
+        }
+        const uint64_t below = __ballot(col < kC && cum < ea.q);
+        const int cls = min(kC - 1, (int)__popcll((below >> ((l >> 4) * 16)) & 0xFFull));
+        if (col == 0 && g < n) {
+          const uint32_t P = ea.products[g];
+          const uint32_t span = ea.right[g] >= ea.left[g] ? ea.right[g] - ea.left[g] + 1 : 0u;
+          const uint32_t ub = min(P, span);
+          uint32_t e = cls < kC - 1 ? ea.thr[cls] : ub;
+          e = max(e, ea.max_b[g]);
+          ea.est[g] = min(e, ub);
+        }
       }
     }
-  }
-  wave_sync();  // this wave's X, H1 and H2 rows are reused by its next group
+    wave_sync();  // this wave's X, H1 and H2 rows are reused by its next group
   }
 }
 
