@@ -159,6 +159,9 @@ def test_training_and_learned_spgemm(gpu):
     assert (st2["decision"]["workflow"] == 1) == (ratio >= 1.5)
     assert st2["decision"]["learned_estimates"] == (ratio >= 1.5)
     log(f"learned rule: products / capacity {ratio:.3f} -> workflow {st2['decision']['workflow']}")
+    if st2["decision"]["workflow"] == 0:
+        # exact counts picked by the predictor: no HyperLogLog sampling
+        assert st2["sampling"]["num_samples"] == 0
     check_product(A, A, dC2.download(), 1e-12, "learned rmat13, default rule")
 
 
