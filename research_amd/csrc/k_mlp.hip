@@ -9,7 +9,8 @@
 // predicted capacity overflow and are recomputed exactly, so C never
 // depends on the prediction.
 //
-// One 256-thread workgroup takes 64 rows; each wave runs 16 rows through
+// A 256-thread workgroup takes 64 rows at a time (inference: persistent,
+// the weights staged once per workgroup); each wave runs 16 rows through
 // the three layers with v_mfma_f32_16x16x32_bf16.  Activations move between
 // layers through LDS (the accumulator layout of one MFMA is not the operand
 // layout of the next).  The backward pass recomputes the forward pass,
